@@ -1,0 +1,187 @@
+"""Benchmark: time to a 1e-6 relative residual on one MI355X.
+
+Workload (default): the 128x128x64 synthetic reservoir grid -- 1,048,576
+cells, 10% inactive, 1% non-neighbour connections, 3x3 blocks
+(943,550 block rows, 6.0M blocks, 2.83M unknowns), b ~ U[-1,1], x0 = 0.
+
+One "step" is one complete solve to rel tol 1e-6 with b and x resident in
+HBM.  Both forms are timed: the plain FP64 ILU0-BiCGSTAB and the precision
+form (FP32 matrix/factor values and inner vectors, FP64 sums, scalars,
+diagonal inverses and refinement).  ``value`` is the precision form's time
+per solve; the FP64 time, iteration counts, the error of each against a
+1e-12 FP64 solve, the roofline of the dominant kernel (SpMV) and the CPU
+oracle baseline are in the same JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="grid128x128x64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md but not built)", flush=True)
+        return 2
+    import torch
+
+    import fpga_amd as fa
+    from fpga_amd.configs import WORKLOADS
+
+    wl = WORKLOADS[args.workload]
+    spec, tol, maxit = wl["spec"], 1e-6, wl["maxit"]
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+
+    t0 = time.perf_counter()
+    s = fa.generate(spec)
+    t_gen = time.perf_counter() - t0
+    log(f"generated {args.workload}: n={s.n} nnzb={s.nnzb} N={s.N} in {t_gen:.2f}s")
+
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_system(s)
+    t0 = time.perf_counter()
+    g.analyse()
+    t_an = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.factor()
+    torch.cuda.synchronize()
+    t_fac = time.perf_counter() - t0
+    log(f"analyse {t_an:.3f}s factor {t_fac:.3f}s colours {g.num_colours}")
+
+    b = torch.from_numpy(s.b).to(dev)
+    x = torch.zeros(s.N, dtype=torch.float64, device=dev)
+
+    # reference solution for the error of both forms (untimed)
+    g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-12, 5000)
+    x_ref = x.clone()
+    nref = torch.linalg.vector_norm(x_ref).item()
+
+    def run_fp64():
+        x.zero_()
+        torch.cuda.synchronize()
+        return g.solve_dev(b.data_ptr(), x.data_ptr(), tol, maxit)
+
+    def run_mixed():
+        x.zero_()
+        torch.cuda.synchronize()
+        return g.solve_mixed_dev(b.data_ptr(), x.data_ptr(), tol, 20, args.inner_tol, maxit, True)
+
+    for _ in range(args.warmup):
+        run_fp64()
+        run_mixed()
+
+    # FP64 form, SpMV launches timed with HIP events on the solver's stream
+    g.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r64 = run_fp64()
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / args.steps
+    err64 = (torch.linalg.vector_norm(x - x_ref).item() / nref)
+    n_spmv, ms_spmv = g.kernel_stats(fa.KIND_SPMV)
+    n_low, ms_low = g.kernel_stats(fa.KIND_LOWER)
+    n_up, ms_up = g.kernel_stats(fa.KIND_UPPER)
+    n_vec, ms_vec = g.kernel_stats(fa.KIND_VEC)
+    g.set_profiling(False)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rmx = run_mixed()
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    errmx = (torch.linalg.vector_norm(x - x_ref).item() / nref)
+
+    spmv_bytes = g.algorithmic_bytes(fa.KIND_SPMV)
+    spmv_ms = ms_spmv / max(n_spmv, 1)
+    spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    low_bytes, up_bytes = g.algorithmic_bytes(fa.KIND_LOWER), g.algorithmic_bytes(fa.KIND_UPPER)
+    iters64 = r64.iterations
+    # algorithmic bytes per FP64 iteration: 2 SpMV + 2 (L + U) + 22 N vector passes
+    it_bytes = 2 * spmv_bytes + 2 * (low_bytes + up_bytes) + 22 * s.N * 8
+    solve_gbs = it_bytes * iters64 / t_fp64 / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, s.values)
+        o.factor()
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            xo, ro = o.solve(s.b, tol=tol, maxit=maxit)
+            times.append(time.perf_counter() - t0)
+        cpu = {"value": min(times), "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"full FP64 solve of {args.workload} to rel 1e-6 ({ro['iterations']} iterations), "
+                         f"best of 2, OpenMP over rows"}
+        log(f"cpu oracle: {times} s, iterations {ro['iterations']}")
+
+    out = {
+        "metric": "time-to-1e-6 rel-res (s), mixed vs FP64 on GPU; achieved HBM GB/s",
+        "value": t_mixed,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_mixed * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 values/vectors, f64 sums+refinement",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "grid": [spec.nx, spec.ny, spec.nz], "block": s.bs,
+                   "block_rows": s.n, "blocks": s.nnzb, "unknowns": s.N, "colours": g.num_colours,
+                   "tol": tol, "inner_tol": args.inner_tol},
+        "fp64": {"time_s": t_fp64, "iterations": iters64, "rel_residual": r64.rel_residual,
+                 "x_rel_err_vs_1e-12": err64, "iters_per_s": iters64 / t_fp64,
+                 "solve_GBs_algorithmic": solve_gbs},
+        "mixed": {"time_s": t_mixed, "outer": rmx.outer_iterations, "inner_iterations": rmx.iterations,
+                  "rel_residual": rmx.rel_residual, "x_rel_err_vs_1e-12": errmx,
+                  "speedup_vs_fp64": t_fp64 / t_mixed},
+        "kernels_ms_avg": {"spmv": spmv_ms, "lower_sweep": ms_low / max(n_low, 1),
+                           "upper_sweep": ms_up / max(n_up, 1), "vector_updates_per_iter": ms_vec / max(n_vec, 1)},
+        "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stream<SPMV> FP64",
+                     "algorithmic_bytes_per_launch": spmv_bytes},
+        "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

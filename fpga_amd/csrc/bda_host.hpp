@@ -1,0 +1,87 @@
+// Host-side data structures shared by the generator, the analysis (colouring,
+// colour-ordered reordering, factor patterns, transposes, tiling) and the
+// device driver.  Plain C++17, no HIP types.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bda {
+
+// Streaming kernels process "tiles": runs of consecutive block rows that
+// belong to one colour, hold at most kTileRows rows and at most
+// tile_cap(bs) blocks, so that one workgroup can stage the tile's per-block
+// partial sums in LDS.
+constexpr int kTileRows = 64;
+inline int tile_cap(int bs) { return bs == 3 ? 640 : 480; }
+
+struct Bcsr {
+    int bs = 0;
+    int n = 0;
+    std::vector<int32_t> rowptr;   // n+1
+    std::vector<int32_t> colidx;   // nnzb, sorted per row
+    int nnzb() const { return rowptr.empty() ? 0 : rowptr.back(); }
+};
+
+// A block pattern stored on the device, with the index of the source block
+// each entry copies from (and whether it is transposed).
+struct MatPattern {
+    int n = 0;
+    std::vector<int32_t> rowptr, colidx;
+    std::vector<int32_t> src;        // source block index (meaning depends on the matrix)
+    std::vector<int32_t> tile_row;   // tile boundaries (row indices), size ntiles+1
+    std::vector<int32_t> colour_tile;// first tile of each colour, size ncolours+1
+    int nnzb() const { return rowptr.empty() ? 0 : rowptr.back(); }
+    int ntiles() const { return tile_row.empty() ? 0 : (int)tile_row.size() - 1; }
+};
+
+struct Analysis {
+    int n = 0, bs = 0, ncolours = 0;
+    bool level_scheduling = false;
+    std::vector<int32_t> colour;      // per original row
+    std::vector<int32_t> perm;        // perm[k] = original row at permuted position k
+    std::vector<int32_t> iperm;       // iperm[orig] = permuted position
+    std::vector<int32_t> colour_ptr;  // ncolours+1
+    MatPattern A;    // permuted A; src = original block index
+    std::vector<int32_t> diag;        // position of the diagonal block of each permuted row in A
+    MatPattern L;    // strictly lower part of permuted A; src = block index in A
+    MatPattern U;    // strictly upper part; src = block index in A
+    MatPattern At;   // transpose of permuted A; src = block index in A (transposed)
+    MatPattern Ut;   // transpose of U (lower); src = block index in U (transposed)
+    MatPattern Lt;   // transpose of L (upper); src = block index in L (transposed)
+    int max_row_blocks = 0;
+};
+
+// First-fit greedy colouring in natural row order on the symmetrised block
+// graph (each row takes the smallest colour absent among its already
+// coloured neighbours).  level_scheduling selects level sets of the
+// lower-triangular graph instead (level = 1 + max level of neighbours j < i).
+// Returns the number of colours.
+int colour_rows(const Bcsr &m, bool level_scheduling, std::vector<int32_t> &colour);
+
+// Full analysis; throws std::runtime_error on a bad pattern.
+void analyse(const Bcsr &m, bool level_scheduling, Analysis &an);
+
+// Build tiles for pattern p whose rows are grouped by colour_ptr.
+void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs);
+
+// Synthetic grid --------------------------------------------------------
+struct GridSpec {
+    int nx = 0, ny = 0, nz = 0, bs = 3;
+    uint64_t seed = 0;
+    double perm_sigma = 1.0, layer_corr = 0.0, inactive_frac = 0.0;
+    double nnc_frac = 0.0;
+    int nnc_dist = 3;
+    double pert_max = 0.2, upwind = 0.3, diag_noise = 0.05;
+};
+
+struct Grid {
+    Bcsr m;
+    std::vector<double> values;   // nnzb * bs * bs
+    std::vector<double> b;        // n * bs
+};
+
+void generate_grid(const GridSpec &spec, Grid &g);
+
+}  // namespace bda

@@ -1,0 +1,1111 @@
+// Handle, device memory, drivers and the extern "C" boundary of the MI355X
+// ILU0-BiCGSTAB solver.  See include/bda_bicgstab.h for the interface.
+//
+// Lifecycle mirrors the reference host library (fpga_functions_bicgstab.cpp):
+// buffers are allocated once and grown only when a larger system arrives
+// (:84-110, :502-505); each solve runs the whole iteration loop on the device
+// (:1305-1325) and returns a decoded status record (:890-934).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/bda_bicgstab.h"
+#include "bda_host.hpp"
+#include "kernels.hpp"
+
+namespace bda {
+namespace {
+
+#define HIP_CHECK(expr)                                                                           \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw std::runtime_error(std::string(#expr) + " failed: " + hipGetErrorString(e_));   \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void ensure(size_t cnt) {
+        if (cnt <= n && p) return;
+        release();
+        const size_t bytes = std::max<size_t>(cnt, 1) * sizeof(T);
+        HIP_CHECK(hipMalloc(&p, bytes));
+        HIP_CHECK(hipMemset(p, 0, bytes));
+        n = std::max<size_t>(cnt, 1);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct DevMat {
+    int n = 0;
+    int64_t nnzb = 0;
+    int ntiles = 0;
+    DBuf<int32_t> rowptr, colidx, tile_row, src;
+    std::vector<int32_t> colour_tile;
+    void upload(const MatPattern &p, hipStream_t s) {
+        n = p.n;
+        nnzb = p.nnzb();
+        ntiles = p.ntiles();
+        colour_tile = p.colour_tile;
+        rowptr.ensure(p.rowptr.size());
+        colidx.ensure(std::max<size_t>(p.colidx.size(), 1));
+        src.ensure(std::max<size_t>(p.src.size(), 1));
+        tile_row.ensure(p.tile_row.size());
+        HIP_CHECK(hipMemcpyAsync(rowptr.p, p.rowptr.data(), p.rowptr.size() * 4, hipMemcpyHostToDevice, s));
+        if (!p.colidx.empty())
+            HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
+        if (!p.src.empty())
+            HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(tile_row.p, p.tile_row.data(), p.tile_row.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+    void release() {
+        rowptr.release();
+        colidx.release();
+        tile_row.release();
+        src.release();
+    }
+};
+
+template <typename TX>
+struct Krylov {
+    DBuf<TX> rt, r, p, phat, v, s, shat, t, y, x;
+    void ensure(size_t n) {
+        for (DBuf<TX> *b : {&rt, &r, &p, &phat, &v, &s, &shat, &t, &y, &x}) b->ensure(n);
+    }
+    void release() {
+        for (DBuf<TX> *b : {&rt, &r, &p, &phat, &v, &s, &shat, &t, &y, &x}) b->release();
+    }
+};
+
+// The matrices one BiCGSTAB run uses: the operator and the two sweeps of its
+// preconditioner.
+template <typename TV>
+struct OpSet {
+    const DevMat *A;
+    const TV *Aval;
+    const DevMat *S1;  // first sweep, colours ascending
+    const TV *S1val;
+    bool s1_dinv;
+    const DevMat *S2;  // second sweep, colours descending
+    const TV *S2val;
+    bool s2_dinv;
+    const double *dinv;
+};
+
+enum ProfKind { PK_SPMV = 0, PK_LOWER = 1, PK_UPPER = 2, PK_VEC = 3, PK_N = 4 };
+
+}  // namespace
+}  // namespace bda
+
+struct bda_handle_s {
+    int bs = 0, n = 0;
+    int64_t nnzb_max = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // host copies
+    bda::Bcsr pattern;
+    bool have_pattern = false, analysed = false, factored = false, have_values = false;
+    bda::Analysis an;
+    std::vector<int32_t> blkrow;  // block row of every permuted A block
+
+    // device matrices
+    bda::DevMat A, L, U, At, Ut, Lt;
+    bda::DBuf<int32_t> d_perm, d_diag, d_blkrow, d_bad;
+    bda::DBuf<double> d_orig, A64, L64, U64, LU, Dinv, DinvT, At64, Ut64, Lt64;
+    bda::DBuf<float> A32, L32, U32;
+    uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX;
+
+    // vectors
+    bda::Krylov<double> k64;
+    bda::Krylov<float> k32;
+    bda::DBuf<double> bp, xo, ro, vin, vout, gl, gx, ggrad;
+    bool last_solve_valid = false;
+    int last_kind = 0;  // 0 plain / adjoint on k64, 1 mixed (residual in ro)
+
+    // reductions and scalar state (0 = solve, 1 = auxiliary)
+    bda::DBuf<double> partials;
+    int32_t partial_cap = 0;
+    bda::DBuf<bda::SolverState> st;
+    bda::SolverState *h_st = nullptr;  // pinned
+    int32_t *h_flag = nullptr;          // pinned poll slots
+    hipEvent_t poll_ev[4] = {};
+
+    // options
+    double timeout_s = 0.0;
+    int abs_tol = 0;
+    int poll_iters = 1;
+    int variant = 1;  // lane layout of the streaming kernel
+
+    // profiling
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> ev_kind;
+    std::vector<int> ev_tag;  // half-iteration a launch belongs to (-1: set-up)
+    int prof_tag = -1;
+    int64_t prof_count[bda::PK_N] = {};
+    double prof_ms[bda::PK_N] = {};
+
+    size_t N() const { return (size_t)n * bs; }
+    size_t Npad() const { return (size_t)n * bs + 2; }
+};
+
+namespace bda {
+namespace {
+
+using H = bda_handle_s;
+
+int blocks_for(int64_t work, int tpb) { return (int)((work + tpb - 1) / tpb); }
+
+void prof_begin(H &h, int kind) {
+    if (!h.profiling) return;
+    if (h.ev_used == h.ev_pool.size()) {
+        hipEvent_t a, b;
+        HIP_CHECK(hipEventCreate(&a));
+        HIP_CHECK(hipEventCreate(&b));
+        h.ev_pool.emplace_back(a, b);
+    }
+    h.ev_kind.resize(h.ev_used + 1);
+    h.ev_tag.resize(h.ev_used + 1);
+    h.ev_kind[h.ev_used] = kind;
+    h.ev_tag[h.ev_used] = h.prof_tag;
+    HIP_CHECK(hipEventRecord(h.ev_pool[h.ev_used].first, h.stream));
+}
+void prof_end(H &h) {
+    if (!h.profiling) return;
+    HIP_CHECK(hipEventRecord(h.ev_pool[h.ev_used].second, h.stream));
+    h.ev_used++;
+}
+// Launches queued after the device-side stop became no-ops; only those of
+// half-iterations up to max_tag did work and are counted.
+void prof_collect(H &h, int max_tag = INT_MAX) {
+    if (!h.profiling || h.ev_used == 0) return;
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    for (size_t i = 0; i < h.ev_used; ++i) {
+        if (h.ev_tag[i] > max_tag) continue;
+        float ms = 0.f;
+        HIP_CHECK(hipEventElapsedTime(&ms, h.ev_pool[i].first, h.ev_pool[i].second));
+        h.prof_count[h.ev_kind[i]]++;
+        h.prof_ms[h.ev_kind[i]] += ms;
+    }
+    h.ev_used = 0;
+}
+
+void check_launch() { HIP_CHECK(hipGetLastError()); }
+
+RedArgs red(H &h, int state_idx, int fin) {
+    RedArgs ra;
+    ra.partials = h.partials.p;
+    ra.cap = h.partial_cap;
+    ra.st = h.st.p + state_idx;
+    ra.fin = fin;
+    return ra;
+}
+
+// grid-wide sum of the partials of the kernel just launched + scalar epilogue
+void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
+    if (nred == 1)
+        hipLaunchKernelGGL((k_finalize<1>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+    else
+        hipLaunchKernelGGL((k_finalize<2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+    check_launch();
+}
+
+// --- streaming kernel launch over the tiles of one colour (or all) ---------
+template <int BS, int MODE, typename TV, typename TX>
+void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
+                   const double *dinv, const TX *w0, int red_self, RedArgs ra, bool use_state, TX *out2 = nullptr,
+                   TX *out3 = nullptr) {
+    StreamArgs<TV, TX> a;
+    a.tile_row = m.tile_row.p;
+    if (colour < 0) {
+        a.tile0 = 0;
+        a.ntiles = m.ntiles;
+    } else {
+        a.tile0 = m.colour_tile[colour];
+        a.ntiles = m.colour_tile[colour + 1] - m.colour_tile[colour];
+    }
+    if (a.ntiles <= 0) return;
+    a.rowptr = m.rowptr.p;
+    a.colidx = m.colidx.p;
+    a.val = val;
+    a.x = x;
+    a.rhs = rhs;
+    a.out = out;
+    a.out2 = out2;
+    a.out3 = out3;
+    a.dinv = dinv;
+    a.w0 = w0;
+    a.red_self = red_self;
+    a.nred = (w0 ? 1 : 0) + (red_self ? 1 : 0);
+    a.ra = ra;
+    if (!use_state) a.ra.st = nullptr;
+    if (a.nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
+    if (a.nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
+    if (h.variant == 0)
+        hipLaunchKernelGGL((k_stream<BS, MODE, TV, TX, 0>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    else
+        hipLaunchKernelGGL((k_stream<BS, MODE, TV, TX, 1>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    check_launch();
+    if (a.nred > 0) launch_finalize(h, a.nred, a.ra, a.ntiles);
+}
+
+int vec_blocks(size_t N) { return (int)((N + kVecElemsPerBlock - 1) / kVecElemsPerBlock); }
+
+// z = M^{-1} p through the two colour sweeps of op
+template <int BS, typename TV, typename TX>
+void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_state) {
+    const int nc = h.an.ncolours;
+    RedArgs ra = red(h, 0, FIN_NONE);
+    prof_begin(h, PK_LOWER);
+    for (int c = 0; c < nc; ++c) {
+        if (op.s1_dinv)
+            launch_stream<BS, SM_SUB_DINV, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, op.dinv, nullptr, 0, ra,
+                                                   use_state);
+        else
+            launch_stream<BS, SM_SUB, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, nullptr, nullptr, 0, ra, use_state);
+    }
+    prof_end(h);
+    prof_begin(h, PK_UPPER);
+    for (int c = nc - 1; c >= 0; --c) {
+        if (op.s2_dinv)
+            launch_stream<BS, SM_SUB_DINV, TV, TX>(h, *op.S2, op.S2val, c, z, y, z, op.dinv, nullptr, 0, ra,
+                                                   use_state);
+        else
+            launch_stream<BS, SM_SUB, TV, TX>(h, *op.S2, op.S2val, c, z, y, z, nullptr, nullptr, 0, ra, use_state);
+    }
+    prof_end(h);
+}
+
+struct RunOut {
+    SolverState st;
+    double ms = 0.0;
+    bool aborted = false;
+};
+
+void init_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
+    SolverState s;
+    std::memset(&s, 0, sizeof(s));
+    s.tol = tol;
+    s.maxit = maxit;
+    s.abs_mode = abs_mode ? 1 : 0;
+    s.norm_slot = 1;
+    *h.h_st = s;
+    HIP_CHECK(hipMemcpyAsync(h.st.p + idx, h.h_st, sizeof(SolverState), hipMemcpyHostToDevice, h.stream));
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
+// One BiCGSTAB run following the reference microprogram (solver_pkg.vhd:120-141).
+// zero_x0: x0 = 0 and the initial residual is the right-hand side itself.
+template <int BS, typename TV, typename TX>
+RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs, bool zero_x0, double tol,
+                    int maxit, bool abs_mode) {
+    const size_t N = h.N();
+    const int vb = vec_blocks(N);
+    init_state(h, 0, tol, maxit, abs_mode);
+    RunOut out;
+    const auto t0 = std::chrono::steady_clock::now();
+    h.prof_tag = -1;
+    if (zero_x0) {
+        hipLaunchKernelGGL((k_init_zero_x0<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, rhs,
+                           k.rt.p, k.r.p, k.p.p, k.x.p, red(h, 0, FIN_INIT));
+        check_launch();
+        launch_finalize(h, 1, red(h, 0, FIN_INIT), vb);
+    } else {
+        if constexpr (std::is_same<TX, double>::value) {
+            prof_begin(h, PK_SPMV);
+            launch_stream<BS, SM_SUB, TV, TX>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr, nullptr, 1,
+                                              red(h, 0, FIN_INIT), true, k.r.p, k.p.p);
+            prof_end(h);
+        } else {
+            throw std::runtime_error("non-zero start vector needs FP64 vectors");
+        }
+    }
+    std::deque<int> pending;
+    int slot = 0;
+    bool stop = false;
+    const int poll = std::max(1, h.poll_iters);
+    for (int it = 1; it <= maxit + 1 && !stop; ++it) {
+        h.prof_tag = 2 * (it - 1);
+        precond<BS, TV, TX>(h, op, k.p.p, k.y.p, k.phat.p, true);
+        prof_begin(h, PK_SPMV);
+        launch_stream<BS, SM_SPMV, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p, 0,
+                                           red(h, 0, FIN_ALPHA), true);
+        prof_end(h);
+        prof_begin(h, PK_VEC);
+        hipLaunchKernelGGL((k_update_xs<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
+                           k.phat.p, k.r.p, k.v.p, k.s.p, red(h, 0, FIN_HALF1));
+        check_launch();
+        launch_finalize(h, 1, red(h, 0, FIN_HALF1), vb);
+        prof_end(h);
+        h.prof_tag = 2 * (it - 1) + 1;
+        precond<BS, TV, TX>(h, op, k.s.p, k.y.p, k.shat.p, true);
+        prof_begin(h, PK_SPMV);
+        launch_stream<BS, SM_SPMV, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p, 1,
+                                           red(h, 0, FIN_OMEGA), true);
+        prof_end(h);
+        prof_begin(h, PK_VEC);
+        hipLaunchKernelGGL((k_update_xr<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
+                           k.shat.p, k.s.p, k.t.p, k.r.p, k.rt.p, red(h, 0, FIN_HALF2));
+        check_launch();
+        launch_finalize(h, 2, red(h, 0, FIN_HALF2), vb);
+        hipLaunchKernelGGL((k_update_p<TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.p.p, k.r.p,
+                           k.v.p, (const SolverState *)(h.st.p));
+        check_launch();
+        prof_end(h);
+        if (it % poll == 0 || it == maxit + 1) {
+            HIP_CHECK(hipMemcpyAsync(&h.h_flag[slot], &h.st.p[0].done, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                     h.stream));
+            HIP_CHECK(hipEventRecord(h.poll_ev[slot], h.stream));
+            pending.push_back(slot);
+            slot = (slot + 1) & 3;
+            while (pending.size() > 1) {
+                const int s = pending.front();
+                pending.pop_front();
+                HIP_CHECK(hipEventSynchronize(h.poll_ev[s]));
+                if (h.h_flag[s]) {
+                    stop = true;
+                    break;
+                }
+            }
+            if (h.timeout_s > 0.0) {
+                const double el =
+                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (el > h.timeout_s) {
+                    out.aborted = true;
+                    stop = true;
+                }
+            }
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    HIP_CHECK(hipMemcpy(h.h_st, h.st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+    out.st = *h.h_st;
+    prof_collect(h, out.st.no_results ? -1 : out.st.h);
+    h.prof_tag = -1;
+    return out;
+}
+
+void fill_result(const RunOut &o, bda_result *res) {
+    if (!res) return;
+    std::memset(res, 0, sizeof(*res));
+    const SolverState &s = o.st;
+    res->half_iterations = s.h;
+    res->iterations = s.h / 2.0 + 0.5;
+    res->last_norm_idx = s.last_norm_idx;
+    for (int i = 0; i < 4; ++i) res->norms[i] = s.norms[i];
+    res->final_norm = s.last_norm;
+    res->rel_residual = s.norms[0] > 0 ? s.last_norm / s.norms[0] : 0.0;
+    res->converged = s.converged;
+    res->no_results = s.no_results;
+    res->max_iter_reached = s.max_iter;
+    res->breakdown = s.breakdown;
+    res->nan_detected = s.nan_flag;
+    res->aborted = o.aborted ? 1 : 0;
+    res->solve_ms = o.ms;
+}
+
+// ----------------------------------------------------------------------------
+void require_analysed(H &h) {
+    if (!h.analysed) throw std::runtime_error("bda_analyse has not been called");
+}
+void require_factored(H &h) {
+    require_analysed(h);
+    if (!h.factored) throw std::runtime_error("bda_factor has not been called");
+}
+
+template <int BS, typename TD>
+void gather_blocks(H &h, int64_t nblk, const int32_t *idx, const double *src, TD *dst, int trans) {
+    if (nblk <= 0) return;
+    const int64_t work = nblk * BS * BS;
+    hipLaunchKernelGGL((k_gather_blocks<BS, TD>), dim3(blocks_for(work, 256)), dim3(256), 0, h.stream, nblk, idx,
+                       src, dst, trans);
+    check_launch();
+}
+
+template <int BS>
+void permute_values(H &h) {
+    gather_blocks<BS, double>(h, h.A.nnzb, h.A.src.p, h.d_orig.p, h.A64.p, 0);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
+void do_analyse(H &h, bool level) {
+    if (!h.have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
+    analyse(h.pattern, level, h.an);
+    const Analysis &an = h.an;
+    h.A.upload(an.A, h.stream);
+    h.L.upload(an.L, h.stream);
+    h.U.upload(an.U, h.stream);
+    h.At.upload(an.At, h.stream);
+    h.Ut.upload(an.Ut, h.stream);
+    h.Lt.upload(an.Lt, h.stream);
+    h.d_perm.ensure(h.n);
+    h.d_diag.ensure(h.n);
+    HIP_CHECK(hipMemcpy(h.d_perm.p, an.perm.data(), h.n * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(h.d_diag.p, an.diag.data(), h.n * 4, hipMemcpyHostToDevice));
+    h.blkrow.resize(an.A.nnzb());
+    for (int k = 0; k < h.n; ++k)
+        for (int32_t q = an.A.rowptr[k]; q < an.A.rowptr[k + 1]; ++q) h.blkrow[q] = k;
+    h.d_blkrow.ensure(h.blkrow.size());
+    HIP_CHECK(hipMemcpy(h.d_blkrow.p, h.blkrow.data(), h.blkrow.size() * 4, hipMemcpyHostToDevice));
+
+    const size_t bb = (size_t)h.bs * h.bs;
+    const int64_t nnzb = an.A.nnzb();
+    h.A64.ensure(nnzb * bb);
+    h.LU.ensure(nnzb * bb);
+    h.L64.ensure(std::max<int64_t>(an.L.nnzb(), 1) * bb);
+    h.U64.ensure(std::max<int64_t>(an.U.nnzb(), 1) * bb);
+    h.Dinv.ensure((size_t)h.n * bb);
+    h.DinvT.ensure((size_t)h.n * bb);
+    h.d_bad.ensure(1);
+    // reduction partials: one slot pair per workgroup of the largest reducing launch
+    const int cap = std::max({h.A.ntiles, h.At.ntiles, vec_blocks(h.N()), 1});
+    h.partial_cap = cap;
+    h.partials.ensure((size_t)2 * cap);
+    h.k64.ensure(h.Npad());
+    h.bp.ensure(h.Npad());
+    h.xo.ensure(h.Npad());
+    h.ro.ensure(h.Npad());
+    h.vin.ensure(h.Npad());
+    h.vout.ensure(h.Npad());
+    h.analysed = true;
+    h.factored = false;
+    if (h.have_values) {
+        if (h.bs == 3) permute_values<3>(h);
+        else permute_values<4>(h);
+    }
+}
+
+template <int BS>
+void do_factor(H &h) {
+    require_analysed(h);
+    if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
+    constexpr int BB = BS * BS;
+    const Analysis &an = h.an;
+    HIP_CHECK(hipMemcpyAsync(h.LU.p, h.A64.p, (size_t)h.A.nnzb * BB * 8, hipMemcpyDeviceToDevice, h.stream));
+    const int32_t big = INT_MAX;
+    HIP_CHECK(hipMemcpyAsync(h.d_bad.p, &big, 4, hipMemcpyHostToDevice, h.stream));
+    for (int c = 0; c < an.ncolours; ++c) {
+        const int r0 = an.colour_ptr[c], r1 = an.colour_ptr[c + 1];
+        hipLaunchKernelGGL((k_bilu0_rows<BS>), dim3(blocks_for(r1 - r0, 128)), dim3(128), 0, h.stream, r0, r1,
+                           h.A.rowptr.p, h.A.colidx.p, h.d_diag.p, h.LU.p, h.Dinv.p, h.d_bad.p);
+        check_launch();
+    }
+    int32_t bad = 0;
+    HIP_CHECK(hipMemcpyAsync(&bad, h.d_bad.p, 4, hipMemcpyDeviceToHost, h.stream));
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    if (bad != INT_MAX)
+        throw std::runtime_error("singular diagonal block in ILU0 at colour-ordered row " + std::to_string(bad) +
+                                 " (original row " + std::to_string(an.perm[bad]) + ")");
+    gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
+    gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
+    gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.factored = true;
+    h.factor_gen++;
+}
+
+template <int BS>
+void ensure_mixed(H &h) {
+    require_factored(h);
+    if (h.mixed_gen == h.factor_gen) return;
+    const size_t bb = (size_t)BS * BS;
+    h.A32.ensure(h.A.nnzb * bb);
+    h.L32.ensure(std::max<int64_t>(h.L.nnzb, 1) * bb);
+    h.U32.ensure(std::max<int64_t>(h.U.nnzb, 1) * bb);
+    gather_blocks<BS, float>(h, h.A.nnzb, nullptr, h.A64.p, h.A32.p, 0);
+    gather_blocks<BS, float>(h, h.L.nnzb, nullptr, h.L64.p, h.L32.p, 0);
+    gather_blocks<BS, float>(h, h.U.nnzb, nullptr, h.U64.p, h.U32.p, 0);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.mixed_gen = h.factor_gen;
+}
+
+template <int BS>
+void ensure_adjoint(H &h) {
+    require_factored(h);
+    if (h.adj_gen == h.factor_gen) return;
+    const size_t bb = (size_t)BS * BS;
+    h.At64.ensure(std::max<int64_t>(h.At.nnzb, 1) * bb);
+    h.Ut64.ensure(std::max<int64_t>(h.Ut.nnzb, 1) * bb);
+    h.Lt64.ensure(std::max<int64_t>(h.Lt.nnzb, 1) * bb);
+    gather_blocks<BS, double>(h, h.At.nnzb, h.At.src.p, h.A64.p, h.At64.p, 1);
+    gather_blocks<BS, double>(h, h.Ut.nnzb, h.Ut.src.p, h.U64.p, h.Ut64.p, 1);
+    gather_blocks<BS, double>(h, h.Lt.nnzb, h.Lt.src.p, h.L64.p, h.Lt64.p, 1);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.adj_gen = h.factor_gen;
+}
+
+OpSet<double> op_fwd64(H &h) {
+    return OpSet<double>{&h.A, h.A64.p, &h.L, h.L64.p, false, &h.U, h.U64.p, true, h.Dinv.p};
+}
+OpSet<float> op_fwd32(H &h) {
+    return OpSet<float>{&h.A, h.A32.p, &h.L, h.L32.p, false, &h.U, h.U32.p, true, h.Dinv.p};
+}
+OpSet<double> op_adj64(H &h) {
+    return OpSet<double>{&h.At, h.At64.p, &h.Ut, h.Ut64.p, true, &h.Lt, h.Lt64.p, false, h.DinvT.p};
+}
+
+void gather_vec(H &h, const double *in, double *out) {
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)h.N(), 256)), dim3(256), 0, h.stream, h.n, h.bs,
+                       h.d_perm.p, in, out);
+    check_launch();
+}
+void scatter_vec(H &h, const double *in, double *out) {
+    hipLaunchKernelGGL(k_scatter_rows, dim3(blocks_for((int64_t)h.N(), 256)), dim3(256), 0, h.stream, h.n, h.bs,
+                       h.d_perm.p, in, out);
+    check_launch();
+}
+
+void check_solve_args(double tol, int maxit) {
+    if (!(tol >= 0.0) || std::isinf(tol)) throw std::runtime_error("tolerance must be finite and >= 0");
+    if (maxit < 0 || maxit > 32767) throw std::runtime_error("maxit must be in [0, 32767]");
+}
+
+// plain FP64 solve on device vectors in original ordering
+template <int BS>
+void solve_dev(H &h, const double *d_b, double *d_x, double tol, int maxit, bda_result *res, bool adjoint) {
+    require_factored(h);
+    check_solve_args(tol, maxit);
+    if (adjoint) ensure_adjoint<BS>(h);
+    gather_vec(h, d_b, h.bp.p);
+    gather_vec(h, d_x, h.k64.x.p);
+    OpSet<double> op = adjoint ? op_adj64(h) : op_fwd64(h);
+    RunOut o = run_bicgstab<BS, double, double>(h, op, h.k64, h.bp.p, false, tol, maxit, h.abs_tol != 0);
+    scatter_vec(h, h.k64.x.p, d_x);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    fill_result(o, res);
+    h.last_solve_valid = true;
+    h.last_kind = 0;
+}
+
+// precision form: FP64 refinement around FP32-value inner solves
+template <int BS, typename TX>
+void mixed_inner(H &h, Krylov<TX> &k, double inner_tol, int inner_maxit, RunOut &o) {
+    OpSet<float> op = op_fwd32(h);
+    o = run_bicgstab<BS, float, TX>(h, op, k, h.ro.p, true, inner_tol, inner_maxit, false);
+    hipLaunchKernelGGL((k_add_correction<TX>), dim3(blocks_for((int64_t)h.N(), kVecThreads)), dim3(kVecThreads), 0,
+                       h.stream, (int64_t)h.N(), h.xo.p, (const TX *)k.x.p);
+    check_launch();
+}
+
+template <int BS>
+void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_outer, double inner_tol,
+                     int inner_maxit, int vec32, bda_result *res) {
+    require_factored(h);
+    check_solve_args(tol, inner_maxit);
+    check_solve_args(inner_tol, 0);
+    if (max_outer < 0) throw std::runtime_error("max_outer must be >= 0");
+    ensure_mixed<BS>(h);
+    if (vec32) h.k32.ensure(h.Npad());
+    const auto t0 = std::chrono::steady_clock::now();
+    gather_vec(h, d_b, h.bp.p);
+    gather_vec(h, d_x, h.xo.p);
+    OpSet<double> op64 = op_fwd64(h);
+    bda_result r;
+    std::memset(&r, 0, sizeof(r));
+    double nrm0 = 0.0, thr = 0.0;
+    int slot = 1;
+    for (int outer = 0;; ++outer) {
+        // FP64 true residual ro = b - A x
+        init_state(h, 1, 0.0, 0, false);
+        prof_begin(h, PK_SPMV);
+        launch_stream<BS, SM_SUB, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr, nullptr, 1,
+                                                  red(h, 1, FIN_STORE), true);
+        prof_end(h);
+        SolverState s1;
+        HIP_CHECK(hipMemcpyAsync(h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost, h.stream));
+        HIP_CHECK(hipStreamSynchronize(h.stream));
+        s1 = *h.h_st;
+        const double nrm = std::sqrt(s1.tot[0]);
+        if (outer == 0) {
+            nrm0 = nrm;
+            r.norms[0] = nrm;
+            const double precision = h.abs_tol ? tol : kAlreadySolved;
+            if (!(nrm > precision)) {
+                r.no_results = 1;
+                if (std::isnan(nrm)) r.nan_detected = 1;
+                break;
+            }
+            thr = h.abs_tol ? tol : tol * nrm;
+        } else {
+            r.norms[slot] = nrm;
+            r.last_norm_idx = slot;
+            slot = slot == 3 ? 1 : slot + 1;
+        }
+        r.final_norm = nrm;
+        if (std::isnan(nrm) || std::isinf(nrm)) {
+            r.nan_detected = 1;
+            break;
+        }
+        if (outer > 0 && !(nrm > thr)) {
+            r.converged = 1;
+            break;
+        }
+        if (outer == max_outer) {
+            r.max_iter_reached = 1;
+            break;
+        }
+        RunOut o;
+        if (vec32) mixed_inner<BS, float>(h, h.k32, inner_tol, inner_maxit, o);
+        else mixed_inner<BS, double>(h, h.k64, inner_tol, inner_maxit, o);
+        r.outer_iterations++;
+        r.inner_half_iterations += o.st.h;
+        r.iterations += o.st.h / 2.0 + 0.5;
+        if (o.st.breakdown || o.st.nan_flag) {
+            r.breakdown = o.st.breakdown;
+            r.nan_detected = o.st.nan_flag;
+            // the correction was still applied; report the FP64 residual after it
+        }
+        if (o.aborted) {
+            r.aborted = 1;
+            break;
+        }
+        if (o.st.no_results) break;  // inner rhs already zero
+    }
+    (void)op64;
+    scatter_vec(h, h.xo.p, d_x);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    r.rel_residual = nrm0 > 0 ? r.final_norm / nrm0 : 0.0;
+    r.half_iterations = r.inner_half_iterations;
+    r.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (res) *res = r;
+    h.last_solve_valid = true;
+    h.last_kind = 1;
+}
+
+template <int BS>
+void spmv_perm(H &h, const double *x, double *y, int transpose) {
+    require_analysed(h);
+    if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
+    if (transpose) {
+        require_factored(h);
+        ensure_adjoint<BS>(h);
+    }
+    HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * 8, hipMemcpyHostToDevice));
+    RedArgs ra = red(h, 1, FIN_NONE);
+    if (transpose)
+        launch_stream<BS, SM_SPMV, double, double>(h, h.At, h.At64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr, nullptr,
+                                                   0, ra, false);
+    else
+        launch_stream<BS, SM_SPMV, double, double>(h, h.A, h.A64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr, nullptr, 0,
+                                                   ra, false);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * 8, hipMemcpyDeviceToHost));
+}
+
+template <int BS>
+void precond_perm(H &h, const double *p, double *z, int transpose) {
+    require_factored(h);
+    if (transpose) ensure_adjoint<BS>(h);
+    HIP_CHECK(hipMemcpy(h.vin.p, p, h.N() * 8, hipMemcpyHostToDevice));
+    OpSet<double> op = transpose ? op_adj64(h) : op_fwd64(h);
+    precond<BS, double, double>(h, op, h.vin.p, h.k64.y.p, h.vout.p, false);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    prof_collect(h);
+    HIP_CHECK(hipMemcpy(z, h.vout.p, h.N() * 8, hipMemcpyDeviceToHost));
+}
+
+template <int BS>
+void grad_dev(H &h, const double *d_lam, const double *d_x, double *d_grad) {
+    require_analysed(h);
+    h.gl.ensure(h.Npad());
+    h.gx.ensure(h.Npad());
+    gather_vec(h, d_lam, h.gl.p);
+    gather_vec(h, d_x, h.gx.p);
+    const int64_t work = h.A.nnzb * BS * BS;
+    hipLaunchKernelGGL((k_grad_blocks<BS>), dim3(blocks_for(work, 256)), dim3(256), 0, h.stream, h.A.nnzb,
+                       h.d_blkrow.p, h.A.colidx.p, h.A.src.p, h.gl.p, h.gx.p, d_grad);
+    check_launch();
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
+template <int BS>
+double time_spmv(H &h, int reps, int fp32) {
+    require_analysed(h);
+    if (fp32) ensure_mixed<BS>(h);
+    RedArgs ra = red(h, 1, FIN_NONE);
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    if (fp32) h.k32.ensure(h.Npad());
+    // warm-up launch, then the timed ones
+    for (int rep = -1; rep < reps; ++rep) {
+        if (rep == 0) HIP_CHECK(hipEventRecord(e0, h.stream));
+        if (fp32) {
+            launch_stream<BS, SM_SPMV, float, float>(h, h.A, h.A32.p, -1, h.k32.phat.p, nullptr, h.k32.v.p, nullptr,
+                                                     nullptr, 0, ra, false);
+        } else {
+            launch_stream<BS, SM_SPMV, double, double>(h, h.A, h.A64.p, -1, h.k64.phat.p, nullptr, h.k64.v.p,
+                                                       nullptr, nullptr, 0, ra, false);
+        }
+    }
+    HIP_CHECK(hipEventRecord(e1, h.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return reps > 0 ? ms / reps : 0.0;
+}
+
+template <typename F>
+int guarded(bda_handle h, F &&f) {
+    if (!h) return 2;
+    try {
+        h->err.clear();
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess && dev != h->device) HIP_CHECK(hipSetDevice(h->device));
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return 1;
+    } catch (...) {
+        h->err = "unknown error";
+        return 1;
+    }
+}
+
+}  // namespace
+}  // namespace bda
+
+using namespace bda;
+
+#define BS_CALL(h, fn, ...)                   \
+    do {                                      \
+        if ((h)->bs == 3) fn<3>(__VA_ARGS__); \
+        else fn<4>(__VA_ARGS__);              \
+    } while (0)
+
+extern "C" {
+
+int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
+    if (!out) return 2;
+    *out = nullptr;
+    if ((bs != 3 && bs != 4) || n <= 0 || nnzb_max < n) return 2;
+    bda_handle h = new (std::nothrow) bda_handle_s();
+    if (!h) return 1;
+    h->bs = bs;
+    h->n = n;
+    h->nnzb_max = nnzb_max;
+    try {
+        HIP_CHECK(hipGetDevice(&h->device));
+        HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->st.ensure(2);
+        HIP_CHECK(hipHostMalloc((void **)&h->h_st, sizeof(SolverState), hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc((void **)&h->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+        for (auto &e : h->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->d_orig.ensure((size_t)nnzb_max * bs * bs);
+        if (const char *v = std::getenv("BDA_STREAM_VARIANT")) h->variant = std::atoi(v) ? 1 : 0;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "ERROR: bda_create: %s\n", e.what());
+        bda_destroy(h);
+        return 1;
+    }
+    *out = h;
+    return 0;
+}
+
+int bda_destroy(bda_handle h) {
+    if (!h) return 0;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (DevMat *m : {&h->A, &h->L, &h->U, &h->At, &h->Ut, &h->Lt}) m->release();
+    for (DBuf<int32_t> *b : {&h->d_perm, &h->d_diag, &h->d_blkrow, &h->d_bad}) b->release();
+    for (DBuf<double> *b : {&h->d_orig, &h->A64, &h->L64, &h->U64, &h->LU, &h->Dinv, &h->DinvT, &h->At64, &h->Ut64,
+                            &h->Lt64, &h->bp, &h->xo, &h->ro, &h->vin, &h->vout, &h->gl, &h->gx, &h->ggrad,
+                            &h->partials})
+        b->release();
+    for (DBuf<float> *b : {&h->A32, &h->L32, &h->U32}) b->release();
+    h->k64.release();
+    h->k32.release();
+    h->st.release();
+    if (h->h_st) (void)hipHostFree(h->h_st);
+    if (h->h_flag) (void)hipHostFree(h->h_flag);
+    for (auto &e : h->poll_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &pr : h->ev_pool) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+const char *bda_last_error(bda_handle h) { return h ? h->err.c_str() : "null handle"; }
+
+int bda_query(bda_handle h, bda_limits *lim) {
+    return guarded(h, [&] {
+        if (!lim) throw std::runtime_error("null limits");
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, h->device));
+        lim->max_block_size = 4;
+        lim->max_blocks_per_row = tile_cap(h->bs);
+        lim->max_iterations = 32767;
+        lim->rows_per_tile = kTileRows;
+        lim->device_memory_bytes = (int64_t)prop.totalGlobalMem;
+        lim->compute_units = prop.multiProcessorCount;
+        lim->wavefront_size = prop.warpSize;
+    });
+}
+
+int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx, const double *values) {
+    return guarded(h, [&] {
+        if (!rowptr || !colidx || !values) throw std::runtime_error("null array");
+        const int64_t nnzb = rowptr[h->n];
+        if (nnzb > h->nnzb_max)
+            throw std::runtime_error("nnzb " + std::to_string(nnzb) + " exceeds nnzb_max " +
+                                     std::to_string(h->nnzb_max));
+        if (rowptr[0] != 0) throw std::runtime_error("rowptr[0] must be 0");
+        h->pattern.bs = h->bs;
+        h->pattern.n = h->n;
+        h->pattern.rowptr.assign(rowptr, rowptr + h->n + 1);
+        h->pattern.colidx.assign(colidx, colidx + nnzb);
+        h->have_pattern = true;
+        h->analysed = false;
+        h->factored = false;
+        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        h->have_values = true;
+    });
+}
+
+int bda_update_values(bda_handle h, const double *values) {
+    return guarded(h, [&] {
+        if (!h->have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
+        const int64_t nnzb = h->pattern.nnzb();
+        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        h->have_values = true;
+        h->factored = false;
+        if (h->analysed) BS_CALL(h, permute_values, *h);
+    });
+}
+
+int bda_update_values_dev(bda_handle h, const double *d_values) {
+    return guarded(h, [&] {
+        if (!h->have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
+        const int64_t nnzb = h->pattern.nnzb();
+        HIP_CHECK(hipMemcpy(h->d_orig.p, d_values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyDeviceToDevice));
+        h->have_values = true;
+        h->factored = false;
+        if (h->analysed) BS_CALL(h, permute_values, *h);
+    });
+}
+
+int bda_analyse(bda_handle h, int32_t level_scheduling) {
+    return guarded(h, [&] { do_analyse(*h, level_scheduling != 0); });
+}
+
+int bda_num_colours(bda_handle h, int32_t *nc) {
+    return guarded(h, [&] {
+        require_analysed(*h);
+        *nc = h->an.ncolours;
+    });
+}
+
+int bda_get_colouring(bda_handle h, int32_t *colour, int32_t *perm, int32_t *colour_ptr) {
+    return guarded(h, [&] {
+        require_analysed(*h);
+        if (colour) std::memcpy(colour, h->an.colour.data(), h->n * 4);
+        if (perm) std::memcpy(perm, h->an.perm.data(), h->n * 4);
+        if (colour_ptr) std::memcpy(colour_ptr, h->an.colour_ptr.data(), (h->an.ncolours + 1) * 4);
+    });
+}
+
+int bda_factor(bda_handle h) {
+    return guarded(h, [&] { BS_CALL(h, do_factor, *h); });
+}
+
+int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t poll_iters) {
+    return guarded(h, [&] {
+        h->timeout_s = timeout_s > 0 ? timeout_s : 0.0;
+        h->abs_tol = abs_tol != 0;
+        h->poll_iters = poll_iters > 0 ? poll_iters : 1;
+    });
+}
+
+int bda_solve_dev(bda_handle h, const double *d_b, double *d_x, double tol, int32_t maxit, bda_result *res) {
+    return guarded(h, [&] { BS_CALL(h, solve_dev, *h, d_b, d_x, tol, maxit, res, false); });
+}
+
+int bda_solve(bda_handle h, const double *b, double *x, double tol, int32_t maxit, bda_result *res) {
+    return guarded(h, [&] {
+        if (!b || !x) throw std::runtime_error("null vector");
+        require_factored(*h);
+        HIP_CHECK(hipMemcpy(h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        BS_CALL(h, solve_dev, *h, h->vin.p, h->vout.p, tol, maxit, res, false);
+        HIP_CHECK(hipMemcpy(x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_get_residual(bda_handle h, double *r) {
+    return guarded(h, [&] {
+        if (!h->last_solve_valid) throw std::runtime_error("no solve has run");
+        const double *src;
+        if (h->last_kind == 1) {
+            src = h->ro.p;
+        } else {
+            HIP_CHECK(hipMemcpy(h->h_st, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+            const int lh = h->h_st->last_half;
+            src = lh == 1 ? h->k64.s.p : lh == 2 ? h->k64.r.p : h->k64.rt.p;
+        }
+        scatter_vec(*h, src, h->vout.p);
+        HIP_CHECK(hipStreamSynchronize(h->stream));
+        HIP_CHECK(hipMemcpy(r, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_solve_mixed_dev(bda_handle h, const double *d_b, double *d_x, double tol, int32_t max_outer,
+                        double inner_tol, int32_t inner_maxit, int32_t vec32, bda_result *res) {
+    return guarded(h, [&] {
+        BS_CALL(h, solve_mixed_dev, *h, d_b, d_x, tol, max_outer, inner_tol, inner_maxit, vec32, res);
+    });
+}
+
+int bda_solve_mixed(bda_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
+                    int32_t inner_maxit, int32_t vec32, bda_result *res) {
+    return guarded(h, [&] {
+        if (!b || !x) throw std::runtime_error("null vector");
+        require_factored(*h);
+        HIP_CHECK(hipMemcpy(h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        BS_CALL(h, solve_mixed_dev, *h, h->vin.p, h->vout.p, tol, max_outer, inner_tol, inner_maxit, vec32, res);
+        HIP_CHECK(hipMemcpy(x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_adjoint_solve_dev(bda_handle h, const double *d_w, double *d_lambda, double tol, int32_t maxit,
+                          bda_result *res) {
+    return guarded(h, [&] { BS_CALL(h, solve_dev, *h, d_w, d_lambda, tol, maxit, res, true); });
+}
+
+int bda_adjoint_solve(bda_handle h, const double *w, double *lambda, double tol, int32_t maxit, bda_result *res) {
+    return guarded(h, [&] {
+        if (!w || !lambda) throw std::runtime_error("null vector");
+        require_factored(*h);
+        HIP_CHECK(hipMemcpy(h->vin.p, w, h->N() * 8, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->vout.p, lambda, h->N() * 8, hipMemcpyHostToDevice));
+        BS_CALL(h, solve_dev, *h, h->vin.p, h->vout.p, tol, maxit, res, true);
+        HIP_CHECK(hipMemcpy(lambda, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_grad_values_dev(bda_handle h, const double *d_lambda, const double *d_x, double *d_grad) {
+    return guarded(h, [&] { BS_CALL(h, grad_dev, *h, d_lambda, d_x, d_grad); });
+}
+
+int bda_grad_values(bda_handle h, const double *lambda, const double *x, double *grad) {
+    return guarded(h, [&] {
+        if (!lambda || !x || !grad) throw std::runtime_error("null vector");
+        require_analysed(*h);
+        const size_t nv = (size_t)h->A.nnzb * h->bs * h->bs;
+        h->ggrad.ensure(nv);
+        HIP_CHECK(hipMemcpy(h->vin.p, lambda, h->N() * 8, hipMemcpyHostToDevice));
+        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        BS_CALL(h, grad_dev, *h, h->vin.p, h->vout.p, h->ggrad.p);
+        HIP_CHECK(hipMemcpy(grad, h->ggrad.p, nv * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_spmv_perm(bda_handle h, const double *x, double *y, int32_t transpose) {
+    return guarded(h, [&] { BS_CALL(h, spmv_perm, *h, x, y, transpose); });
+}
+
+int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t transpose) {
+    return guarded(h, [&] { BS_CALL(h, precond_perm, *h, p, z, transpose); });
+}
+
+int bda_factor_sizes(bda_handle h, int32_t *nl, int32_t *nu) {
+    return guarded(h, [&] {
+        require_analysed(*h);
+        if (nl) *nl = (int32_t)h->an.L.nnzb();
+        if (nu) *nu = (int32_t)h->an.U.nnzb();
+    });
+}
+
+int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *l_values, int32_t *u_rowptr,
+                    int32_t *u_colidx, double *u_values, double *dinv) {
+    return guarded(h, [&] {
+        require_factored(*h);
+        const Analysis &an = h->an;
+        const size_t bb = (size_t)h->bs * h->bs;
+        if (l_rowptr) std::memcpy(l_rowptr, an.L.rowptr.data(), (h->n + 1) * 4);
+        if (l_colidx) std::memcpy(l_colidx, an.L.colidx.data(), an.L.colidx.size() * 4);
+        if (u_rowptr) std::memcpy(u_rowptr, an.U.rowptr.data(), (h->n + 1) * 4);
+        if (u_colidx) std::memcpy(u_colidx, an.U.colidx.data(), an.U.colidx.size() * 4);
+        if (l_values && an.L.nnzb() > 0)
+            HIP_CHECK(hipMemcpy(l_values, h->L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost));
+        if (u_values && an.U.nnzb() > 0)
+            HIP_CHECK(hipMemcpy(u_values, h->U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost));
+        if (dinv) HIP_CHECK(hipMemcpy(dinv, h->Dinv.p, (size_t)h->n * bb * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+int bda_set_profiling(bda_handle h, int32_t on) {
+    return guarded(h, [&] {
+        h->profiling = on != 0;
+        for (int k = 0; k < PK_N; ++k) {
+            h->prof_count[k] = 0;
+            h->prof_ms[k] = 0.0;
+        }
+        h->ev_used = 0;
+    });
+}
+
+int bda_get_kernel_stats(bda_handle h, int32_t kind, int64_t *launches, double *total_ms) {
+    return guarded(h, [&] {
+        if (kind < 0 || kind >= PK_N) throw std::runtime_error("bad kernel kind");
+        if (launches) *launches = h->prof_count[kind];
+        if (total_ms) *total_ms = h->prof_ms[kind];
+    });
+}
+
+int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms) {
+    return guarded(h, [&] {
+        double v = 0.0;
+        if (h->bs == 3) v = time_spmv<3>(*h, reps, fp32);
+        else v = time_spmv<4>(*h, reps, fp32);
+        if (avg_ms) *avg_ms = v;
+    });
+}
+
+int bda_algorithmic_bytes(bda_handle h, int32_t kind, int32_t fp32, double *bytes) {
+    return guarded(h, [&] {
+        require_analysed(*h);
+        const double bb = (double)h->bs * h->bs;
+        const double sv = fp32 ? 4.0 : 8.0, sx = fp32 ? 4.0 : 8.0;
+        const double N = (double)h->N(), n = (double)h->n;
+        double v = 0.0;
+        switch (kind) {
+        case 0: v = (double)h->A.nnzb * (bb * sv + 4.0) + 4.0 * (n + 1) + 2.0 * N * sx; break;
+        case 1: v = (double)h->L.nnzb * (bb * sv + 4.0) + 4.0 * (n + 1) + 2.0 * N * sx; break;
+        case 2: v = (double)h->U.nnzb * (bb * sv + 4.0) + n * bb * 8.0 + 4.0 * (n + 1) + 2.0 * N * sx; break;
+        default: throw std::runtime_error("bad kernel kind");
+        }
+        if (bytes) *bytes = v;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,212 @@
+/*
+ * bda_bicgstab.h -- C ABI of the MI355X ILU0-preconditioned BiCGSTAB solver.
+ *
+ * The reference exposes its solver through a C++-linkage static library
+ * (linearalgebra/ilu0bicgstab/xilinx/alveo_u280/vitis_20192/OPM-integration/makefile:32,63-66)
+ * whose entry points take raw host arrays in a per-colour device format and
+ * OpenCL handles (src/sda_app/common/fpga_functions_bicgstab.hpp:30-137).
+ * The caller (the Flow FPGA backend of opm-simulators) had to colour,
+ * reorder and factorise the matrix itself.  This ABI keeps the reference's
+ * lifecycle -- allocate once at the maximum size, update values per linear
+ * solve without reallocating, run the whole BiCGSTAB loop on the device,
+ * decode a status record -- but takes a plain block-CSR matrix and does
+ * colouring, colour-ordered reordering and block ILU0 itself.
+ *
+ * Conventions (all functions):
+ *   - return 0 on success, non-zero on failure (the reference returns 0/1,
+ *     fpga_functions_bicgstab.cpp:206-216); the message is available from
+ *     bda_last_error();
+ *   - no C++ exception crosses this boundary;
+ *   - host pointers are owned by the caller and copied; device memory is
+ *     owned by the handle; "_dev" entry points take device pointers
+ *     (hipMalloc'ed, e.g. a torch tensor's data_ptr) and are synchronous;
+ *   - one HIP stream per handle, every call blocks until its work is done;
+ *   - a handle must not be used from two threads at once.
+ *
+ * Vectors passed to bda_solve*, bda_adjoint_solve* and bda_grad_values are in
+ * the caller's ORIGINAL row ordering; the handle permutes internally.
+ * bda_spmv_perm / bda_precond_apply_perm / bda_get_factors operate in the
+ * colour-ordered (permuted) numbering and exist for component-level checks.
+ */
+#ifndef BDA_BICGSTAB_H
+#define BDA_BICGSTAB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bda_handle_s *bda_handle;
+
+/* Solve status, mirrors the fields decode_debuginfo_bicgstab() extracts from
+ * the kernel's status line (src/sda_app/common/bicgstab_utils.cpp:90-280,
+ * wrapper bits at bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:75-82),
+ * plus the breakdown / NaN flags the reference does not have. */
+typedef struct bda_result {
+    double iterations;        /* half_iterations/2 + 0.5 (fpga_functions_bicgstab.cpp:922-924) */
+    int32_t half_iterations;  /* the kernel's iteration counter in half steps */
+    int32_t last_norm_idx;    /* newest entry of norms[1..3] */
+    double norms[4];          /* norms[0] = ||r0||, norms[1..3] ring of the last three norms */
+    double final_norm;        /* last recurrence residual norm (||s|| or ||r||) */
+    double rel_residual;      /* final_norm / norms[0] */
+    int32_t converged;        /* stopped because norm <= threshold */
+    int32_t no_results;       /* ||r0|| already below threshold: x untouched (wrapper bit 1) */
+    int32_t max_iter_reached; /* stopped by the iteration cap */
+    int32_t breakdown;        /* rho, v.rt, t.t or omega became exactly zero */
+    int32_t nan_detected;     /* a norm or scalar became NaN/Inf */
+    int32_t aborted;          /* host watchdog fired (wrapper bit 0, param0) */
+    int32_t outer_iterations; /* refinement steps (mixed form), 0 otherwise */
+    int32_t inner_half_iterations; /* total inner half iterations (mixed form) */
+    double solve_ms;          /* wall time of the device solve (setup excluded) */
+} bda_result;
+
+/* Capacity / configuration query: counterpart of fpga_kernel_query()
+ * (fpga_functions_bicgstab.cpp:1332-1452).  There are no hard limits beyond
+ * device memory; the fields report what this build assumes. */
+typedef struct bda_limits {
+    int32_t max_block_size;        /* 4 */
+    int32_t max_blocks_per_row;    /* longest block row the streaming kernels accept */
+    int32_t max_iterations;        /* 32767: 15-bit full-iteration counter */
+    int32_t rows_per_tile;         /* block rows per workgroup tile */
+    int64_t device_memory_bytes;   /* total HBM of the current device */
+    int32_t compute_units;
+    int32_t wavefront_size;
+} bda_limits;
+
+/* Synthetic reservoir-grid generator (7-point stencil, 3x3/4x4 blocks). */
+typedef struct bda_grid_spec {
+    int32_t nx, ny, nz;       /* grid */
+    int32_t bs;               /* block size, 3 or 4 */
+    uint64_t seed;
+    double perm_sigma;        /* log-permeability standard deviation */
+    double layer_corr;        /* share of log-perm variance common to a z layer (0..1) */
+    double inactive_frac;     /* fraction of cells marked inactive and removed */
+    double nnc_frac;          /* fraction of active cells given a non-neighbour connection */
+    int32_t nnc_dist;         /* Chebyshev radius of such connections */
+    double pert_max;          /* off-diagonal block = -T (I + P), P ~ U[0, pert_max] */
+    double upwind;            /* extra -upwind*T*I on the higher-index cell's coupling */
+    double diag_noise;        /* std of the N(0, s) entries added to the diagonal block */
+} bda_grid_spec;
+
+typedef struct bda_grid_s *bda_grid;
+
+/* ---- generator ------------------------------------------------------- */
+int bda_grid_create(const bda_grid_spec *spec, bda_grid *out);
+int bda_grid_sizes(bda_grid g, int32_t *n, int32_t *nnzb, int32_t *bs);
+int bda_grid_copy(bda_grid g, int32_t *rowptr, int32_t *colidx, double *values, double *b);
+int bda_grid_destroy(bda_grid g);
+
+/* Host-only colouring of a block pattern, identical to what bda_analyse
+ * computes; colour_ptr needs room for n+1 entries.  No device is touched. */
+int bda_colour_pattern(int32_t n, const int32_t *rowptr, const int32_t *colidx, int32_t level_scheduling,
+                       int32_t *colour, int32_t *perm, int32_t *colour_ptr, int32_t *num_colours);
+
+/* ---- handle lifecycle -------------------------------------------------- */
+/* Allocate for a system of n block rows and at most nnzb_max blocks of
+ * size bs x bs (bs in {3,4}).  Replaces fpga_setup_host_datamem +
+ * fpga_setup_device_datamem (fpga_functions_bicgstab.cpp:84-496,729-798). */
+int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out);
+int bda_destroy(bda_handle h);
+const char *bda_last_error(bda_handle h);
+int bda_query(bda_handle h, bda_limits *limits);
+
+/* Block-CSR upload: rowptr[n+1], colidx[nnzb] (sorted per row, diagonal
+ * present), values[nnzb*bs*bs] row-major blocks.  Replaces
+ * fpga_copy_host_datamem (fpga_functions_bicgstab.cpp:506-691). */
+int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx,
+                    const double *values);
+/* New values on the uploaded pattern (Newton step); no reallocation. */
+int bda_update_values(bda_handle h, const double *values);
+int bda_update_values_dev(bda_handle h, const double *d_values);
+
+/* Greedy colouring + colour-ordered permutation (host).  Counterpart of the
+ * colour sizes / P-index arrays the reference consumes
+ * (fpga_functions_bicgstab.cpp:606-613).  level_scheduling != 0 selects
+ * level sets of the lower-triangular graph instead of colours (the flag of
+ * fpga_setup_host_datamem, fpga_functions_bicgstab.cpp:84). */
+int bda_analyse(bda_handle h, int32_t level_scheduling);
+int bda_num_colours(bda_handle h, int32_t *num_colours);
+/* colour[n] per original row, perm[n] (perm[k] = original row at colour-
+ * ordered position k), colour_ptr[num_colours+1]. */
+int bda_get_colouring(bda_handle h, int32_t *colour, int32_t *perm, int32_t *colour_ptr);
+
+/* Block ILU0 in colour order on the device (FP64; FP32 copies of the
+ * factors and of A are made for the mixed form).  No reference counterpart:
+ * the reference receives L, U and the inverted diagonal blocks ready-made
+ * (fpga_functions_bicgstab.cpp:615-617,629). */
+int bda_factor(bda_handle h);
+
+/* Host watchdog in seconds (0 = off), counterpart of param0's abort cycles
+ * (bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:605-643); abs_tol != 0
+ * selects the absolute compare of config_bits bit 1 (solver.vhd:637-641). */
+int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t poll_iters);
+
+/* Plain FP64 ILU0-BiCGSTAB, the reference's microprogram
+ * (solver_pkg.vhd:120-141): x holds x0 on entry and the solution on exit.
+ * maxit follows the kernel's max_iter parameter: maxit full iterations are
+ * run, the next one stops after its first half.  Replaces
+ * fpga_set_kernel_parameters + fpga_kernel_run + fpga_copy_from_device_debugbuf
+ * + fpga_map_results (fpga_functions_bicgstab.cpp:1246-1325,890-934,1073-1190). */
+int bda_solve(bda_handle h, const double *b, double *x, double tol, int32_t maxit,
+              bda_result *res);
+int bda_solve_dev(bda_handle h, const double *d_b, double *d_x, double tol, int32_t maxit,
+                  bda_result *res);
+/* The last recurrence residual (s or r) in original ordering (the R1/R2
+ * result buffer of fpga_map_results). */
+int bda_get_residual(bda_handle h, double *r);
+
+/* Precision form: FP32 matrix/factor values, FP32 products, FP64 sums,
+ * FP64 diagonal-block inverses, dots and scalars; inner BiCGSTAB to
+ * inner_tol (relative), FP64 iterative refinement to tol (relative to the
+ * initial FP64 residual).  vec32 != 0 also stores the inner Krylov vectors
+ * in FP32.  No reference counterpart (the reference is FP64 only,
+ * alveo_u280/rtl/pkg/constants.vhd:98-100). */
+int bda_solve_mixed(bda_handle h, const double *b, double *x, double tol, int32_t max_outer,
+                    double inner_tol, int32_t inner_maxit, int32_t vec32, bda_result *res);
+int bda_solve_mixed_dev(bda_handle h, const double *d_b, double *d_x, double tol,
+                        int32_t max_outer, double inner_tol, int32_t inner_maxit,
+                        int32_t vec32, bda_result *res);
+
+/* Adjoint: solve A^T lambda = w with the transposed colour-ordered ILU0
+ * (U^T sweep over colours ascending, then L^T descending).  lambda holds the
+ * start vector on entry.  No reference counterpart. */
+int bda_adjoint_solve(bda_handle h, const double *w, double *lambda, double tol,
+                      int32_t maxit, bda_result *res);
+int bda_adjoint_solve_dev(bda_handle h, const double *d_w, double *d_lambda, double tol,
+                          int32_t maxit, bda_result *res);
+/* grad[nnzb*bs*bs] (original block layout) = -lambda_i x_j^T on every block
+ * (i,j) of the pattern: dL/dA for L(x(A)) with dL/dx = w. */
+int bda_grad_values(bda_handle h, const double *lambda, const double *x, double *grad);
+int bda_grad_values_dev(bda_handle h, const double *d_lambda, const double *d_x,
+                        double *d_grad);
+
+/* ---- component access (colour-ordered numbering) ---------------------- */
+/* y = A x (transpose != 0: y = A^T x). */
+int bda_spmv_perm(bda_handle h, const double *x, double *y, int32_t transpose);
+/* z = M^{-1} p with M = LU (transpose != 0: z = M^{-T} p). */
+int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t transpose);
+/* Factors in the colour-ordered numbering: L and U in the strictly-lower /
+ * strictly-upper block pattern of the permuted matrix (block counts from
+ * bda_factor_sizes), dinv[n*bs*bs] the inverted diagonal blocks. */
+int bda_factor_sizes(bda_handle h, int32_t *nnzb_lower, int32_t *nnzb_upper);
+int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *l_values,
+                    int32_t *u_rowptr, int32_t *u_colidx, double *u_values, double *dinv);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Kernel classes timed with HIP events on the handle's stream while
+ * profiling is on: 0 = SpMV, 1 = lower sweep, 2 = upper sweep, 3 = vector
+ * updates.  Counts and summed milliseconds since the last reset. */
+int bda_set_profiling(bda_handle h, int32_t on);
+int bda_get_kernel_stats(bda_handle h, int32_t kind, int64_t *launches, double *total_ms);
+/* reps back-to-back FP64 (or FP32-value, fp32 != 0) SpMV launches on
+ * device-resident vectors; average launch time in ms from HIP events. */
+int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
+/* Algorithmic bytes of one launch of the kernel classes above (FP64 form). */
+int bda_algorithmic_bytes(bda_handle h, int32_t kind, int32_t fp32, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BDA_BICGSTAB_H */

@@ -1,0 +1,201 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Component kernels (SpMV, block ILU0, L/U sweeps, their transposes) use the
+same per-row summation order as the oracle and must match it bit for bit.
+Solves: iteration counts within one full iteration, x within 1e-8 relative
+at equal iteration counts (FP64).
+"""
+import numpy as np
+import pytest
+
+import fpga_amd as fa
+import oracle
+from fpga_amd.configs import GRID_32x32x16, GRID_128x128x64, scaled
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "grid32x32x16": GRID_32x32x16,
+    "grid20x18x10_nnc": scaled(GRID_128x128x64, 20, 18, 10),
+    "grid12x10x6_bs4_nnc": fa.GridSpec(12, 10, 6, bs=4, seed=3, perm_sigma=2.0, layer_corr=0.5,
+                                       inactive_frac=0.1, nnc_frac=0.05, nnc_dist=3),
+}
+
+
+def _pair(spec, level=False):
+    print(f"case {spec}", flush=True)
+    s = fa.generate(spec)
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_system(s)
+    g.analyse(level)
+    g.factor()
+    o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, s.values, level)
+    o.factor()
+    return s, g, o
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def system(request):
+    s, g, o = _pair(CASES[request.param])
+    yield request.param, s, g, o
+    g.close()
+
+
+def test_colouring_bit_exact(system):
+    _, s, g, o = system
+    for a, b in zip(g.colouring(), o.colouring()):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_spmv_bit_exact(system):
+    _, s, g, o = system
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, s.N)
+    np.testing.assert_array_equal(g.spmv_perm(x), o.spmv_perm(x))
+    np.testing.assert_array_equal(g.spmv_perm(x, transpose=True), o.spmv_perm(x, transpose=True))
+
+
+def test_factors_bit_exact(system):
+    _, s, g, o = system
+    (glr, glc, glv), (gur, guc, guv), gd = g.factors()
+    (olr, olc, olv), (our, ouc, ouv), od = o.factors()
+    np.testing.assert_array_equal(glr, olr)
+    np.testing.assert_array_equal(glc, olc)
+    np.testing.assert_array_equal(gur, our)
+    np.testing.assert_array_equal(guc, ouc)
+    np.testing.assert_array_equal(glv, olv)
+    np.testing.assert_array_equal(guv, ouv)
+    np.testing.assert_array_equal(gd, od)
+
+
+def test_precond_bit_exact(system):
+    _, s, g, o = system
+    rng = np.random.default_rng(2)
+    p = rng.uniform(-1, 1, s.N)
+    np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
+    np.testing.assert_array_equal(g.precond_apply_perm(p, transpose=True), o.precond_perm(p, transpose=True))
+
+
+@pytest.mark.parametrize("maxit", [0, 1, 2, 5])
+def test_fixed_iterations(system, maxit):
+    """tol = 0 never converges: both stop at the same half step; x agrees."""
+    _, s, g, o = system
+    xg, rg = g.solve(s.b, tol=0.0, maxit=maxit)
+    xo, ro = o.solve(s.b, tol=0.0, maxit=maxit)
+    assert rg.max_iter_reached and ro["max_iter_reached"]
+    assert rg.half_iterations == ro["half_iterations"] == 2 * maxit
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-10 * np.abs(xo).max())
+    np.testing.assert_allclose(rg.norms[0], ro["norms"][0], rtol=1e-13)
+
+
+def test_solve_parity(system):
+    name, s, g, o = system
+    tol = 1e-2 if name == "grid32x32x16" else 1e-6
+    xg, rg = g.solve(s.b, tol=tol, maxit=200)
+    xo, ro = o.solve(s.b, tol=tol, maxit=200)
+    print(name, rg.half_iterations, ro["half_iterations"], rg.rel_residual, ro["rel_residual"])
+    assert rg.converged and ro["converged"]
+    assert abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    if rg.half_iterations == ro["half_iterations"]:
+        err = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+        assert err < 1e-8, err
+    assert rg.last_norm_idx == ro["last_norm_idx"] or rg.half_iterations != ro["half_iterations"]
+
+
+def test_residual_vector(system):
+    _, s, g, o = system
+    xg, rg = g.solve(s.b, tol=1e-6, maxit=200)
+    r = g.residual()
+    assert np.isclose(np.linalg.norm(r), rg.final_norm, rtol=1e-12)
+
+
+def test_mixed_precision(system):
+    name, s, g, o = system
+    x64, _ = o.solve(s.b, tol=1e-12, maxit=1000)
+    for vec32 in (False, True):
+        xm, rm = g.solve_mixed(s.b, tol=1e-6, inner_tol=1e-4, vec32=vec32)
+        xom, rom = o.solve_mixed(s.b, tol=1e-6, inner_tol=1e-4, vec32=vec32)
+        assert rm.converged and rom["converged"]
+        err = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+        print(name, "vec32" if vec32 else "vec64", rm.outer_iterations, rm.inner_half_iterations, err)
+        assert err < 1e-5
+        assert abs(rm.outer_iterations - rom["outer_iterations"]) <= 1
+
+
+def test_adjoint_parity(system):
+    name, s, g, o = system
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(s.N)
+    lg, rg = g.adjoint_solve(w, tol=1e-10, maxit=500)
+    lo, ro = o.adjoint_solve(w, tol=1e-10, maxit=500)
+    assert rg.converged and ro["converged"]
+    assert abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    err = np.linalg.norm(lg - lo) / np.linalg.norm(lo)
+    assert err < 1e-8, err
+    x, _ = o.solve(s.b, tol=1e-12, maxit=500)
+    np.testing.assert_array_equal(g.grad_values(lo, x), o.grad(lo, x))
+
+
+def test_level_scheduling_solve():
+    s, g, o = _pair(scaled(GRID_128x128x64, 12, 10, 8), level=True)
+    for a, b in zip(g.colouring(), o.colouring()):
+        np.testing.assert_array_equal(a, b)
+    xg, rg = g.solve(s.b, tol=1e-8, maxit=200)
+    xo, ro = o.solve(s.b, tol=1e-8, maxit=200)
+    assert rg.converged and abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    g.close()
+
+
+def test_edge_zero_rhs():
+    s, g, o = _pair(scaled(GRID_32x32x16, 6, 5, 4))
+    x, r = g.solve(np.zeros(s.N), tol=1e-6, maxit=10)
+    assert r.no_results and not r.converged and r.half_iterations == 0
+    assert np.all(x == 0)
+    x0 = np.ones(s.N)
+    x, r = g.solve(np.zeros(s.N), x0=x0, tol=1e-6, maxit=10)
+    assert not r.no_results  # b - A x0 != 0
+    g.close()
+
+
+def test_edge_abs_tolerance():
+    s, g, o = _pair(scaled(GRID_32x32x16, 6, 5, 4))
+    g.set_options(abs_tol=True)
+    x, r = g.solve(s.b, tol=1e-3, maxit=100)
+    assert r.converged and r.final_norm <= 1e-3
+    big = g.solve(s.b, tol=1e9, maxit=100)[1]
+    assert big.no_results
+    g.close()
+
+
+def test_edge_nan_values():
+    s = fa.generate(scaled(GRID_32x32x16, 6, 5, 4))
+    vals = s.values.copy()
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_bcsr(s.rowptr, s.colidx, vals)
+    g.analyse()
+    g.factor()
+    b = s.b.copy()
+    b[7] = np.nan
+    x, r = g.solve(b, tol=1e-6, maxit=10)
+    assert r.nan_detected and not r.converged
+    g.close()
+
+
+def test_update_values_reuses_handle():
+    spec = scaled(GRID_32x32x16, 8, 8, 4)
+    s = fa.generate(spec)
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_system(s)
+    g.analyse()
+    g.factor()
+    x1, _ = g.solve(s.b, tol=1e-10, maxit=200)
+    v2 = s.values * 1.5
+    g.update_values(v2)
+    g.factor()
+    x2, r2 = g.solve(s.b, tol=1e-10, maxit=200)
+    o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, v2)
+    o.factor()
+    xo, _ = o.solve(s.b, tol=1e-10, maxit=200)
+    assert np.linalg.norm(x2 - xo) / np.linalg.norm(xo) < 1e-8
+    assert np.linalg.norm(x2 * 1.5 - x1) / np.linalg.norm(x1) < 1e-8
+    g.close()
