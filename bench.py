@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--mode", choices=["full", "kernels"], default="full",
+                    help="kernels: only repeated SpMV launches (FP64 and FP32 values) and a "
+                         "reference copy of known size, for rocprofv3 counter runs")
+    ap.add_argument("--reps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -79,6 +83,9 @@ def main():
 
     b = torch.from_numpy(s.b).to(dev)
     x = torch.zeros(s.N, dtype=torch.float64, device=dev)
+
+    if args.mode == "kernels":
+        return kernels_only(args, g, s, torch, dev, fa)
 
     # reference solution for the error of both forms (untimed)
     g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-12, 5000)
@@ -179,6 +186,33 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    g.close()
+    return 0
+
+
+def kernels_only(args, g, s, torch, dev, fa):
+    """Repeated SpMV launches for profiling; prints one JSON line."""
+    res = {"workload": args.workload, "reps": args.reps}
+    for fp32 in (False, True):
+        ms = g.time_spmv(args.reps, fp32)
+        by = g.algorithmic_bytes(fa.KIND_SPMV, fp32)
+        res["spmv_fp32" if fp32 else "spmv_fp64"] = {"avg_ms": ms, "algorithmic_bytes": by,
+                                                     "GBs": by / (ms * 1e-3) / 1e9}
+    # reference copy of known size (64 Mi doubles = 512 MiB read + 512 MiB written)
+    src = torch.ones(1 << 26, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.reps
+    res["copy_512MiB"] = {"avg_ms": ms, "GBs": 2 * src.numel() * 8 / (ms * 1e-3) / 1e9}
+    print(json.dumps(res), flush=True)
     g.close()
     return 0
 

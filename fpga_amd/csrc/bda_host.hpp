@@ -14,7 +14,7 @@ namespace bda {
 // tile_cap(bs) blocks, so that one workgroup can stage the tile's per-block
 // partial sums in LDS.
 constexpr int kTileRows = 64;
-inline int tile_cap(int bs) { return bs == 3 ? 640 : 480; }
+inline int tile_cap(int bs) { return bs == 3 ? 168 : 128; }
 
 struct Bcsr {
     int bs = 0;

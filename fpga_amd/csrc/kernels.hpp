@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bda_host.hpp"  // kTileRows
+
 namespace bda {
 
 // ---------------------------------------------------------------------------
@@ -39,6 +41,7 @@ struct SolverState {
     int32_t done;
     int32_t converged, no_results, max_iter, breakdown, nan_flag;
     int32_t last_half;      // 1: s is the latest residual, 2: r is
+    int32_t *host_done;     // mapped host memory: set to 1 when done is set
 };
 
 enum Fin : int32_t {
@@ -72,6 +75,11 @@ __device__ inline void push_norm(SolverState *st, double nrm) {
 
 __device__ inline bool finite(double v) { return !__builtin_isnan(v) && !__builtin_isinf(v); }
 
+__device__ inline void signal_done(SolverState *st) {
+    st->done = 1;
+    if (st->host_done) __hip_atomic_store(st->host_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Scalar epilogues; run by one thread after a grid-wide reduction.
 __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) {
     switch (fin) {
@@ -92,7 +100,7 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         if (!(nrm > precision)) {
             st->no_results = 1;
             if (!finite(nrm)) st->nan_flag = 1;
-            st->done = 1;
+            signal_done(st);
         } else {
             st->thr = st->abs_mode ? st->tol : st->tol * nrm;
         }
@@ -101,7 +109,7 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
     case FIN_ALPHA:    // step 5: divide rho_new by the dot result
         if (t0 == 0.0 || !finite(t0)) {
             if (t0 == 0.0) st->breakdown = 1; else st->nan_flag = 1;
-            st->done = 1;
+            signal_done(st);
         } else {
             st->alpha = st->rho / t0;
         }
@@ -110,16 +118,16 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         const double nrm = sqrt(t0);
         push_norm(st, nrm);
         st->last_half = 1;
-        if (!finite(nrm)) { st->nan_flag = 1; st->done = 1; }
-        else if (!(nrm > st->thr)) { st->converged = 1; st->done = 1; }
-        else if ((st->h >> 1) == st->maxit) { st->max_iter = 1; st->done = 1; }
+        if (!finite(nrm)) { st->nan_flag = 1; signal_done(st); }
+        else if (!(nrm > st->thr)) { st->converged = 1; signal_done(st); }
+        else if ((st->h >> 1) == st->maxit) { st->max_iter = 1; signal_done(st); }
         else st->h += 1;
         break;
     }
     case FIN_OMEGA:    // step 10
         if (t1 == 0.0 || !finite(t0) || !finite(t1)) {
             if (t1 == 0.0) st->breakdown = 1; else st->nan_flag = 1;
-            st->done = 1;
+            signal_done(st);
         } else {
             st->omega = t0 / t1;
         }
@@ -128,13 +136,13 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         const double nrm = sqrt(t0);
         push_norm(st, nrm);
         st->last_half = 2;
-        if (!finite(nrm)) { st->nan_flag = 1; st->done = 1; }
-        else if (!(nrm > st->thr)) { st->converged = 1; st->done = 1; }
+        if (!finite(nrm)) { st->nan_flag = 1; signal_done(st); }
+        else if (!(nrm > st->thr)) { st->converged = 1; signal_done(st); }
         else {
             st->h += 1;
             st->rho_prev = st->rho;
             st->rho = t1;
-            if (t1 == 0.0 || st->omega == 0.0) { st->breakdown = 1; st->done = 1; }
+            if (t1 == 0.0 || st->omega == 0.0) { st->breakdown = 1; signal_done(st); }
             else st->beta = (st->rho / st->rho_prev) * (st->alpha / st->omega);
         }
         break;
@@ -181,15 +189,31 @@ constexpr int kFinThreads = 1024;
 
 template <int NRED>
 __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts) {
+    constexpr int U = 16;  // independent loads in flight per thread
     __shared__ double sred[kFinThreads / 64][2];
     if (ra.st->done) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc[NRED];
 #pragma unroll
     for (int j = 0; j < NRED; ++j) {
-        acc[j] = 0.0;
-        for (int i = threadIdx.x; i < nparts; i += kFinThreads) acc[j] += ra.partials[(size_t)j * ra.cap + i];
-        acc[j] = wave_sum(acc[j]);
+        const double *pp = ra.partials + (size_t)j * ra.cap;
+        double a8[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) a8[u] = 0.0;
+        for (int i0 = 0; i0 < nparts; i0 += kFinThreads * U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = i0 + u * kFinThreads + threadIdx.x;
+                v[u] = i < nparts ? pp[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) a8[u] += v[u];
+        }
+        double t = a8[0];
+#pragma unroll
+        for (int u = 1; u < U; ++u) t += a8[u];
+        acc[j] = wave_sum(t);
         if (lane == 0) sred[wave][j] = acc[j];
     }
     __syncthreads();
@@ -211,10 +235,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts
 // value array.  Phase 1 streams the tile's values with one lane per matrix
 // entry (7 blocks of 3x3 = 63 lanes, or 4 blocks of 4x4 = 64 lanes per wave
 // step), so every wave load instruction reads consecutive memory; each lane
-// gathers its x entry, multiplies, and the BS lanes of one block row combine
-// their products left to right with cross-lane shuffles.  The per-block row
-// partials go to LDS.  Phase 2 gives one thread per scalar row, which adds
-// its block partials in column order and applies the epilogue.
+// gathers its x entry (column indices staged in LDS), multiplies, and the
+// BS lanes of one block row combine their products left to right with
+// cross-lane shuffles.  The per-block row partials go to LDS.  Phase 2 gives
+// one thread per scalar row, which adds its block partials in column order
+// and applies the epilogue.
 //
 // Summation order per scalar row r of block row i:
 //   sum = 0;  for blocks j in column order: sum += ((a_r0 x_0 + a_r1 x_1) + a_r2 x_2)
@@ -222,20 +247,20 @@ enum StreamMode : int { SM_SPMV = 0, SM_SUB = 1, SM_SUB_DINV = 2 };
 
 template <typename TV, typename TX>
 struct StreamArgs {
-    const int32_t *tile_row;
+    const int2 *tile;   // tile descriptors {first row, first block}, ntiles_total + 1
     int32_t tile0, ntiles;
     const int32_t *rowptr;
     const int32_t *colidx;
     const TV *val;
-    const TX *x;        // vector multiplied (gathered through colidx)
+    const TX *x;        // vector multiplied (gathered through colidx) ...
+    const TX *x_lo;     // ... except columns below split, which come from x_lo
+    int32_t split;
     const TX *rhs;      // SUB modes: out = rhs - A x
     TX *out;
     TX *out2;           // optional copies of out (initial residual -> r, p)
     TX *out3;
     const double *dinv; // SUB_DINV: out = dinv * (rhs - A x)
-    const TX *w0;       // reduction 0: sum out * w0   (if non-null)
-    int32_t red_self;   // reduction 1 (or 0 if w0 null): sum out * out
-    int32_t nred;
+    const TX *w0;       // SR_W0 reductions: sum out * w0
     RedArgs ra;
 };
 
@@ -261,122 +286,147 @@ __device__ inline int xcd_tile(int b, int G) {
     return x * q + (x < rem ? x : rem) + i;
 }
 
-// Lane layouts of phase 1:
-//   VAR 0  one lane per matrix entry: a wave step reads 7 (3x3) or 4 (4x4)
-//          whole blocks as one contiguous run of 63/64 values; the BS lanes
-//          of a block row combine their products with cross-lane shuffles.
-//   VAR 1  one lane per block row segment (BS consecutive values): a wave
-//          step covers 21 (3x3) or 16 (4x4) blocks, each lane multiplies its
-//          BS values by the BS gathered x entries on its own.
-// Both add the products of a block row segment left to right, so the
-// result is the same to the bit.
-template <int BS, int MODE, typename TV, typename TX, int VAR>
+// FP64 matrix values are loaded non-temporally; FP32 values are not (measured
+// on MI355X: the hint helps the FP64 stream, costs the FP32 one a few %).
+__device__ inline double load_stream(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ inline float load_stream(const float *p) { return *p; }
+
+// Blocks of padding behind every matrix value and column-index array, so
+// that the streaming kernel's clamped loads never leave the allocation.
+constexpr int kPadBlocks = 8;
+
+// Tile capacity in blocks; the host tiler uses the same numbers.
+template <int BS> struct TileCap;
+template <> struct TileCap<3> { static constexpr int value = 168; };  // 8 workgroup steps of 21 blocks
+template <> struct TileCap<4> { static constexpr int value = 128; };  // 8 workgroup steps of 16 blocks
+
+// Matrix values and column indices are read once per launch; the FP64
+// values and the column indices use non-temporal loads, so that they evict
+// less of the x vector from L2 (in colour order the reuses of an x entry by
+// its z-neighbours lie about one grid layer of streamed blocks apart, about
+// one L2 worth of values).
+//
+// Reductions fused into the epilogue: bit 0 = sum out*w0, bit 1 = sum out*out.
+enum StreamRed : int { SR_NONE = 0, SR_W0 = 1, SR_SELF = 2, SR_BOTH = 3 };
+
+// Phase 1 has no loop: every lane owns a fixed set of S value slots of the
+// tile (S = CAP / blocks per workgroup step) and issues all their loads at
+// once.  The x entries a tile needs are gathered cooperatively into LDS,
+// each (block, component) once, instead of once per matrix entry.
+//
+// All loads are issued without intervening branches (indices are clamped
+// into range and results masked afterwards; the value and column arrays
+// carry a few blocks of padding for this): the compiler's wait-count
+// insertion falls back to waiting for every outstanding load at a branch,
+// which would serialise the memory round trips.  Column indices and row
+// pointers are issued first, so the first barrier waits for them and not
+// for the value stream issued after them.  A tile costs three dependent
+// memory round trips: descriptor; columns/values/row operands; x gathers.
+template <int BS, int MODE, int RED, typename TV, typename TX>
 __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
-    constexpr int CAP = BS == 3 ? 640 : 480;
-    constexpr int UNR = VAR == 0 ? 4 : 2;  // wave steps in flight per lane
+    constexpr int CAP = TileCap<BS>::value;
+    constexpr int BPW = 64 / BB;           // blocks per wave step (7 or 4)
+    constexpr int VPW = BPW * BB;          // lanes carrying a value (63 or 64)
+    constexpr int S = CAP / (NW * BPW);    // value slots per lane
+    constexpr int CJ = (CAP + NT - 1) / NT;       // column loads per thread
+    constexpr int XG = (CAP * BS + NT - 1) / NT;  // x gathers per thread
+    constexpr int NRED = (RED & 1) + ((RED >> 1) & 1);
 
+    // LDS arrays are sized to whole multiples of the workgroup so that their
+    // fills are unconditional (a conditional store lets the compiler sink the
+    // feeding load next to it, where it is waited for at once)
     __shared__ double part[CAP * BS];
-    __shared__ int32_t scol[CAP];
+    __shared__ TX sx[XG * NT];
+    __shared__ int32_t scol[CJ * NT];
+    __shared__ int32_t srow[NT];
     __shared__ double tv[MODE == SM_SUB_DINV ? NT : 1];
 
     if (a.ra.st != nullptr && a.ra.st->done) return;
 
     const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
-    const int r0 = a.tile_row[tile], r1 = a.tile_row[tile + 1];
-    const int b0 = a.rowptr[r0], b1 = a.rowptr[r1];
-    const int nblk = b1 - b0;
-
-    // phase-2 operands that do not depend on phase 1, fetched early
+    const int2 d0 = a.tile[tile], d1 = a.tile[tile + 1];
+    const int r0 = d0.x, r1 = d1.x, b0 = d0.y;
+    const int nblk = d1.y - b0;
+    const int nrow = r1 - r0;
     const int t = threadIdx.x;
-    const int k = t / BS, r = t - k * BS;
-    const int row = r0 + k;
-    const bool has_row = row < r1;
-    int q1 = 0, q2 = 0;
-    double rhs_v = 0.0;
-    double dv[BS];
-    if (has_row) {
-        q1 = a.rowptr[row] - b0;
-        q2 = a.rowptr[row + 1] - b0;
-        if constexpr (MODE != SM_SPMV) rhs_v = (double)a.rhs[(size_t)row * BS + r];
-        if constexpr (MODE == SM_SUB_DINV) {
+    const int lane = t & 63, wave = t >> 6;
+
+    // 1. column indices and row pointers
+    int ci[CJ];
 #pragma unroll
-            for (int c = 0; c < BS; ++c) dv[c] = a.dinv[(size_t)row * BB + r * BS + c];
-        }
+    for (int j = 0; j < CJ; ++j) ci[j] = __builtin_nontemporal_load(&a.colidx[b0 + min(t + j * NT, nblk)]);
+    const int rp = a.rowptr[r0 + min(t, nrow)];
+
+    // 2. values: slot u of this lane is block (u*NW + wave)*BPW + lb, entry e
+    const int lb = lane / BB, e = lane - lb * BB, er = e / BS, ec = e - er * BS;
+    const bool active_lane = lane < VPW;
+    const TV *val = a.val + (size_t)b0 * BB + e;
+    TV av[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        const int bl = (u * NW + wave) * BPW + lb;
+        av[u] = load_stream(&val[(size_t)min(bl, nblk) * BB]);
     }
 
-    for (int i = t; i < nblk; i += NT) scol[i] = a.colidx[b0 + i];
+    // 3. per-row operands of the epilogue
+    const int k = t / BS, r = t - k * BS;
+    const int row = r0 + k;
+    const bool has_row = k < nrow;
+    const size_t gi = (size_t)(r0 + min(k, nrow - 1)) * BS + r;
+    double rhs_v = 0.0, w_v = 0.0;
+    double dv[BS];
+    if constexpr (MODE != SM_SPMV) rhs_v = (double)a.rhs[gi];
+    if constexpr ((RED & SR_W0) != 0) w_v = (double)a.w0[gi];
+    if constexpr (MODE == SM_SUB_DINV) {
+        const double *D = a.dinv + gi * BS;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) dv[c] = D[c];
+    }
+
+#pragma unroll
+    for (int j = 0; j < CJ; ++j) scol[t + j * NT] = ci[j];
+    srow[t] = rp - b0;
     __syncthreads();
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const TV *val = a.val + (size_t)b0 * BB;
-    if constexpr (VAR == 0) {
-        constexpr int BPW = 64 / BB;       // blocks per wave step (7 or 4)
-        constexpr int VPW = BPW * BB;      // lanes carrying a value (63 or 64)
-        const int lb = lane / BB, e = lane - lb * BB, er = e / BS, ec = e - er * BS;
-        const bool active_lane = lane < VPW;
-        for (int q0 = wave * BPW; q0 < nblk; q0 += NW * BPW * UNR) {
-            TV av[UNR];
-            TX xv[UNR];
+    // 4. x entries of the tile, (block, component) pairs laid out like part[]
+    TX xg[XG];
+    const int last = nblk * BS - 1;
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int bl = q0 + u * NW * BPW + lb;
-                const bool ok = active_lane && bl < nblk;
-                av[u] = ok ? val[(size_t)bl * BB + e] : (TV)0;
-                xv[u] = ok ? a.x[(size_t)scol[bl] * BS + ec] : (TX)0;
-            }
+    for (int j = 0; j < XG; ++j) {
+        const int i = min(t + j * NT, last < 0 ? 0 : last);
+        const int q = i / BS, c = i - q * BS;
+        const int cb = nblk > 0 ? scol[q] : 0;
+        xg[j] = (cb < a.split ? a.x_lo : a.x)[(size_t)cb * BS + c];
+    }
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const double p = prod<TV, TX>(av[u], xv[u]);
-                double sm = p;
-                const double p1 = __shfl_down(p, 1);
-                const double p2 = __shfl_down(p, 2);
-                sm = sm + p1;
-                sm = sm + p2;
-                if constexpr (BS == 4) {
-                    const double p3 = __shfl_down(p, 3);
-                    sm = sm + p3;
-                }
-                const int bl = q0 + u * NW * BPW + lb;
-                if (active_lane && ec == 0 && bl < nblk) part[bl * BS + er] = sm;
-            }
+    for (int j = 0; j < XG; ++j) sx[t + j * NT] = xg[j];
+    __syncthreads();
+
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+        const int bl = (u * NW + wave) * BPW + lb;
+        const bool ok = active_lane && bl < nblk;
+        const TX xv = ok ? sx[bl * BS + ec] : (TX)0;
+        const double p = prod<TV, TX>(ok ? av[u] : (TV)0, xv);
+        double sm = p;
+        const double p1 = __shfl_down(p, 1);
+        const double p2 = __shfl_down(p, 2);
+        sm = sm + p1;
+        sm = sm + p2;
+        if constexpr (BS == 4) {
+            const double p3 = __shfl_down(p, 3);
+            sm = sm + p3;
         }
-    } else {
-        constexpr int SPW = 64 / BS;       // blocks per wave step (21 or 16)
-        const int lb = lane / BS, lr = lane - lb * BS;
-        const bool active_lane = lane < SPW * BS;
-        for (int q0 = wave * SPW; q0 < nblk; q0 += NW * SPW * UNR) {
-            TV av[UNR][BS];
-            TX xv[UNR][BS];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int bl = q0 + u * NW * SPW + lb;
-                const bool ok = active_lane && bl < nblk;
-                const int cb = ok ? scol[bl] : 0;
-                const TV *ap = val + ((size_t)bl * BS + lr) * BS;
-                const TX *xp = a.x + (size_t)cb * BS;
-#pragma unroll
-                for (int c = 0; c < BS; ++c) {
-                    av[u][c] = ok ? ap[c] : (TV)0;
-                    xv[u][c] = ok ? xp[c] : (TX)0;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                double sm = prod<TV, TX>(av[u][0], xv[u][0]);
-#pragma unroll
-                for (int c = 1; c < BS; ++c) sm = sm + prod<TV, TX>(av[u][c], xv[u][c]);
-                const int bl = q0 + u * NW * SPW + lb;
-                if (active_lane && bl < nblk) part[bl * BS + lr] = sm;
-            }
-        }
+        if (ok && ec == 0) part[bl * BS + er] = sm;
     }
     __syncthreads();
 
     double res = 0.0;
     if (has_row) {
+        const int q1 = srow[k], q2 = srow[k + 1];
         double sum = 0.0;
         for (int q = q1; q < q2; ++q) sum += part[q * BS + r];
         if constexpr (MODE == SM_SPMV) res = sum;
@@ -394,54 +444,76 @@ __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     }
     double red[2] = {0.0, 0.0};
     if (has_row) {
-        const size_t gi = (size_t)row * BS + r;
         const TX o = from_double<TX>(res);
         a.out[gi] = o;
         if (a.out2) a.out2[gi] = o;
         if (a.out3) a.out3[gi] = o;
         const double od = (double)o;
-        int j = 0;
-        if (a.w0) red[j++] = od * (double)a.w0[gi];
-        if (a.red_self) red[j] = od * od;
+        if constexpr (RED == SR_W0) red[0] = od * w_v;
+        if constexpr (RED == SR_SELF) red[0] = od * od;
+        if constexpr (RED == SR_BOTH) {
+            red[0] = od * w_v;
+            red[1] = od * od;
+        }
     }
-    if (a.nred == 1) {
+    if constexpr (NRED == 1) {
         double v1[1] = {red[0]};
         block_partials<NT, 1>(v1, a.ra);
-    } else if (a.nred == 2) {
+    } else if constexpr (NRED == 2) {
         block_partials<NT, 2>(red, a.ra);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Fused BiCGSTAB vector updates.  Each thread handles pairs of elements;
-// vectors are allocated with at least one element of zero padding so that a
-// pair load at the odd tail stays inside the allocation.
+// Fused BiCGSTAB vector updates.  Each thread handles kVecPairsPerThread
+// element pairs.  Vectors carry kVecPad elements of padding: the pair at an
+// odd tail has padding as its second element, and threads past the end all
+// work on one dummy pair just after the last pair.  Padding is computed like
+// the rest, never read by the matrix kernels and left out of every
+// reduction.  Loads and stores are therefore unconditional, and all loads of
+// a thread are in flight together.
 constexpr int kVecThreads = 256;
 constexpr int kVecPairsPerThread = 4;
 constexpr int kVecElemsPerBlock = kVecThreads * kVecPairsPerThread * 2;
+constexpr int kVecPad = 4;
 
 template <typename T> struct Pair;
 template <> struct Pair<double> { using type = double2; };
 template <> struct Pair<float> { using type = float2; };
 
 template <typename T>
-__device__ inline void ld2(const T *p, size_t i, double &a, double &b) {
-    const typename Pair<T>::type v = *reinterpret_cast<const typename Pair<T>::type *>(p + i);
+__device__ inline void ld2(const T *p, int64_t pi, double &a, double &b) {
+    const typename Pair<T>::type v = reinterpret_cast<const typename Pair<T>::type *>(p)[pi];
     a = (double)v.x;
     b = (double)v.y;
 }
 template <typename T>
-__device__ inline void st2(T *p, size_t i, double a, double b, bool both) {
-    if (both) {
-        typename Pair<T>::type v;
-        v.x = (T)a;
-        v.y = (T)b;
-        *reinterpret_cast<typename Pair<T>::type *>(p + i) = v;
-    } else {
-        p[i] = (T)a;
-    }
+__device__ inline void st2(T *p, int64_t pi, double a, double b) {
+    typename Pair<T>::type v;
+    v.x = (T)a;
+    v.y = (T)b;
+    reinterpret_cast<typename Pair<T>::type *>(p)[pi] = v;
 }
 template <typename T> __device__ inline double rnd(double v) { return (double)(T)v; }
+
+struct VecIdx {
+    int64_t pi[kVecPairsPerThread];   // pair index (the dummy pair past the end)
+    bool ok[kVecPairsPerThread];      // pair exists
+    bool second[kVecPairsPerThread];  // its second element is < N
+};
+__device__ inline VecIdx vec_index(int64_t N) {
+    VecIdx v;
+    const int64_t npairs = (N + 1) / 2;
+    const int64_t base = (int64_t)blockIdx.x * (kVecElemsPerBlock / 2) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        const int64_t p = base + (int64_t)j * kVecThreads;
+        v.ok[j] = p < npairs;
+        v.pi[j] = p < npairs ? p : npairs;
+        v.second[j] = 2 * p + 1 < N;
+    }
+    return v;
+}
 
 // steps 6-8: x += alpha phat; s = r - alpha v; ||s||^2
 template <typename TX, typename TXX>
@@ -449,26 +521,24 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xs(int64_t N, TXX *x, co
                                                           const TX *v, TX *s, RedArgs ra) {
     if (ra.st->done) return;
     const double alpha = ra.st->alpha, malpha = -alpha;
-    double acc = 0.0;
-    const int64_t base = (int64_t)blockIdx.x * (kVecElemsPerBlock / 2);
+    const VecIdx ix = vec_index(N);
+    double x0[kVecPairsPerThread], x1[kVecPairsPerThread], p0[kVecPairsPerThread], p1[kVecPairsPerThread];
+    double r0[kVecPairsPerThread], r1[kVecPairsPerThread], v0[kVecPairsPerThread], v1[kVecPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
-        const int64_t pi = base + threadIdx.x + (int64_t)j * kVecThreads;
-        const int64_t i = 2 * pi;
-        if (i < N) {
-            const bool both = i + 1 < N;
-            double x0, x1, p0, p1, r0, r1, v0, v1;
-            ld2(x, i, x0, x1);
-            ld2(phat, i, p0, p1);
-            ld2(r, i, r0, r1);
-            ld2(v, i, v0, v1);
-            const double nx0 = alpha * p0 + x0, nx1 = alpha * p1 + x1;
-            const double s0 = rnd<TX>(malpha * v0 + r0), s1 = rnd<TX>(malpha * v1 + r1);
-            st2(x, i, nx0, nx1, both);
-            st2(s, i, s0, s1, both);
-            acc += s0 * s0;
-            if (both) acc += s1 * s1;
-        }
+        ld2(x, ix.pi[j], x0[j], x1[j]);
+        ld2(phat, ix.pi[j], p0[j], p1[j]);
+        ld2(r, ix.pi[j], r0[j], r1[j]);
+        ld2(v, ix.pi[j], v0[j], v1[j]);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        const double s0 = rnd<TX>(malpha * v0[j] + r0[j]), s1 = rnd<TX>(malpha * v1[j] + r1[j]);
+        st2(x, ix.pi[j], alpha * p0[j] + x0[j], alpha * p1[j] + x1[j]);
+        st2(s, ix.pi[j], s0, s1);
+        acc += ix.ok[j] ? s0 * s0 : 0.0;
+        acc += ix.second[j] ? s1 * s1 : 0.0;
     }
     double vv[1] = {acc};
     block_partials<kVecThreads, 1>(vv, ra);
@@ -480,31 +550,28 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, co
                                                           const TX *t, TX *r, const TX *rt, RedArgs ra) {
     if (ra.st->done) return;
     const double omega = ra.st->omega, momega = -omega;
-    double acc[2] = {0.0, 0.0};
-    const int64_t base = (int64_t)blockIdx.x * (kVecElemsPerBlock / 2);
+    const VecIdx ix = vec_index(N);
+    double x0[kVecPairsPerThread], x1[kVecPairsPerThread], h0[kVecPairsPerThread], h1[kVecPairsPerThread];
+    double s0[kVecPairsPerThread], s1[kVecPairsPerThread], t0[kVecPairsPerThread], t1[kVecPairsPerThread];
+    double q0[kVecPairsPerThread], q1[kVecPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
-        const int64_t pi = base + threadIdx.x + (int64_t)j * kVecThreads;
-        const int64_t i = 2 * pi;
-        if (i < N) {
-            const bool both = i + 1 < N;
-            double x0, x1, h0, h1, s0, s1, t0, t1, q0, q1;
-            ld2(x, i, x0, x1);
-            ld2(shat, i, h0, h1);
-            ld2(s, i, s0, s1);
-            ld2(t, i, t0, t1);
-            ld2(rt, i, q0, q1);
-            const double nx0 = omega * h0 + x0, nx1 = omega * h1 + x1;
-            const double r0 = rnd<TX>(momega * t0 + s0), r1 = rnd<TX>(momega * t1 + s1);
-            st2(x, i, nx0, nx1, both);
-            st2(r, i, r0, r1, both);
-            acc[0] += r0 * r0;
-            acc[1] += q0 * r0;
-            if (both) {
-                acc[0] += r1 * r1;
-                acc[1] += q1 * r1;
-            }
-        }
+        ld2(x, ix.pi[j], x0[j], x1[j]);
+        ld2(shat, ix.pi[j], h0[j], h1[j]);
+        ld2(s, ix.pi[j], s0[j], s1[j]);
+        ld2(t, ix.pi[j], t0[j], t1[j]);
+        ld2(rt, ix.pi[j], q0[j], q1[j]);
+    }
+    double acc[2] = {0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        const double ra0 = rnd<TX>(momega * t0[j] + s0[j]), ra1 = rnd<TX>(momega * t1[j] + s1[j]);
+        st2(x, ix.pi[j], omega * h0[j] + x0[j], omega * h1[j] + x1[j]);
+        st2(r, ix.pi[j], ra0, ra1);
+        acc[0] += ix.ok[j] ? ra0 * ra0 : 0.0;
+        acc[1] += ix.ok[j] ? q0[j] * ra0 : 0.0;
+        acc[0] += ix.second[j] ? ra1 * ra1 : 0.0;
+        acc[1] += ix.second[j] ? q1[j] * ra1 : 0.0;
     }
     block_partials<kVecThreads, 2>(acc, ra);
 }
@@ -515,20 +582,19 @@ __global__ __launch_bounds__(kVecThreads) void k_update_p(int64_t N, TX *p, cons
                                                          const SolverState *st) {
     if (st->done) return;
     const double beta = st->beta, momega = -st->omega;
-    const int64_t base = (int64_t)blockIdx.x * (kVecElemsPerBlock / 2);
+    const VecIdx ix = vec_index(N);
+    double p0[kVecPairsPerThread], p1[kVecPairsPerThread], r0[kVecPairsPerThread], r1[kVecPairsPerThread];
+    double v0[kVecPairsPerThread], v1[kVecPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
-        const int64_t pi = base + threadIdx.x + (int64_t)j * kVecThreads;
-        const int64_t i = 2 * pi;
-        if (i < N) {
-            const bool both = i + 1 < N;
-            double p0, p1, r0, r1, v0, v1;
-            ld2(p, i, p0, p1);
-            ld2(r, i, r0, r1);
-            ld2(v, i, v0, v1);
-            const double t0 = momega * v0 + p0, t1 = momega * v1 + p1;
-            st2(p, i, beta * t0 + r0, beta * t1 + r1, both);
-        }
+        ld2(p, ix.pi[j], p0[j], p1[j]);
+        ld2(r, ix.pi[j], r0[j], r1[j]);
+        ld2(v, ix.pi[j], v0[j], v1[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        const double t0 = momega * v0[j] + p0[j], t1 = momega * v1[j] + p1[j];
+        st2(p, ix.pi[j], beta * t0 + r0[j], beta * t1 + r1[j]);
     }
 }
 
@@ -538,25 +604,20 @@ template <typename TX, typename TXX>
 __global__ __launch_bounds__(kVecThreads) void k_init_zero_x0(int64_t N, const double *rhs, TX *rt, TX *r, TX *p,
                                                              TXX *x, RedArgs ra) {
     if (ra.st->done) return;
+    const VecIdx ix = vec_index(N);
+    double b0[kVecPairsPerThread], b1[kVecPairsPerThread];
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) ld2(rhs, ix.pi[j], b0[j], b1[j]);
     double acc = 0.0;
-    const int64_t base = (int64_t)blockIdx.x * (kVecElemsPerBlock / 2);
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
-        const int64_t pi = base + threadIdx.x + (int64_t)j * kVecThreads;
-        const int64_t i = 2 * pi;
-        if (i < N) {
-            const bool both = i + 1 < N;
-            double b0, b1;
-            ld2(rhs, i, b0, b1);
-            b0 = rnd<TX>(b0);
-            b1 = rnd<TX>(b1);
-            st2(rt, i, b0, b1, both);
-            st2(r, i, b0, b1, both);
-            st2(p, i, b0, b1, both);
-            st2(x, i, 0.0, 0.0, both);
-            acc += b0 * b0;
-            if (both) acc += b1 * b1;
-        }
+        const double c0 = rnd<TX>(b0[j]), c1 = rnd<TX>(b1[j]);
+        st2(rt, ix.pi[j], c0, c1);
+        st2(r, ix.pi[j], c0, c1);
+        st2(p, ix.pi[j], c0, c1);
+        st2(x, ix.pi[j], 0.0, 0.0);
+        acc += ix.ok[j] ? c0 * c0 : 0.0;
+        acc += ix.second[j] ? c1 * c1 : 0.0;
     }
     double vv[1] = {acc};
     block_partials<kVecThreads, 1>(vv, ra);

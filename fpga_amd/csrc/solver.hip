@@ -57,7 +57,8 @@ struct DevMat {
     int n = 0;
     int64_t nnzb = 0;
     int ntiles = 0;
-    DBuf<int32_t> rowptr, colidx, tile_row, src;
+    DBuf<int32_t> rowptr, colidx, src;
+    DBuf<int2> tile;  // {first row, first block} per tile, plus the end
     std::vector<int32_t> colour_tile;
     void upload(const MatPattern &p, hipStream_t s) {
         n = p.n;
@@ -65,21 +66,23 @@ struct DevMat {
         ntiles = p.ntiles();
         colour_tile = p.colour_tile;
         rowptr.ensure(p.rowptr.size());
-        colidx.ensure(std::max<size_t>(p.colidx.size(), 1));
+        colidx.ensure(p.colidx.size() + kPadBlocks);
         src.ensure(std::max<size_t>(p.src.size(), 1));
-        tile_row.ensure(p.tile_row.size());
+        tile.ensure(p.tile_row.size());
+        std::vector<int2> desc(p.tile_row.size());
+        for (size_t i = 0; i < desc.size(); ++i) desc[i] = make_int2(p.tile_row[i], p.rowptr[p.tile_row[i]]);
         HIP_CHECK(hipMemcpyAsync(rowptr.p, p.rowptr.data(), p.rowptr.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.colidx.empty())
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(tile_row.p, p.tile_row.data(), p.tile_row.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(tile.p, desc.data(), desc.size() * sizeof(int2), hipMemcpyHostToDevice, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
     void release() {
         rowptr.release();
         colidx.release();
-        tile_row.release();
+        tile.release();
         src.release();
     }
 };
@@ -147,14 +150,16 @@ struct bda_handle_s {
     int32_t partial_cap = 0;
     bda::DBuf<bda::SolverState> st;
     bda::SolverState *h_st = nullptr;  // pinned
-    int32_t *h_flag = nullptr;          // pinned poll slots
+    int32_t *h_done = nullptr;          // mapped host flag the device sets when a solve stops
+    int32_t *d_done = nullptr;          // its device address
     hipEvent_t poll_ev[4] = {};
 
     // options
     double timeout_s = 0.0;
     int abs_tol = 0;
     int poll_iters = 1;
-    int variant = 1;  // lane layout of the streaming kernel
+    int refine_adaptive = 1;  // last refinement steps ask only what the outer target needs
+    double refine_eta = 0.5;
 
     // profiling
     bool profiling = false;
@@ -167,11 +172,13 @@ struct bda_handle_s {
     double prof_ms[bda::PK_N] = {};
 
     size_t N() const { return (size_t)n * bs; }
-    size_t Npad() const { return (size_t)n * bs + 2; }
+    size_t Npad() const { return (size_t)n * bs + bda::kVecPad; }
 };
 
 namespace bda {
 namespace {
+
+static_assert(TileCap<3>::value == 168 && TileCap<4>::value == 128, "device tile capacity must match tile_cap()");
 
 using H = bda_handle_s;
 
@@ -232,12 +239,13 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
 }
 
 // --- streaming kernel launch over the tiles of one colour (or all) ---------
-template <int BS, int MODE, typename TV, typename TX>
+template <int BS, int MODE, int RED, typename TV, typename TX>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
-                   const double *dinv, const TX *w0, int red_self, RedArgs ra, bool use_state, TX *out2 = nullptr,
-                   TX *out3 = nullptr) {
+                   const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
+                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0) {
+    constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
-    a.tile_row = m.tile_row.p;
+    a.tile = m.tile.p;
     if (colour < 0) {
         a.tile0 = 0;
         a.ntiles = m.ntiles;
@@ -250,50 +258,64 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     a.colidx = m.colidx.p;
     a.val = val;
     a.x = x;
+    a.x_lo = x_lo ? x_lo : x;
+    a.split = x_lo ? split : 0;
     a.rhs = rhs;
     a.out = out;
     a.out2 = out2;
     a.out3 = out3;
     a.dinv = dinv;
     a.w0 = w0;
-    a.red_self = red_self;
-    a.nred = (w0 ? 1 : 0) + (red_self ? 1 : 0);
     a.ra = ra;
     if (!use_state) a.ra.st = nullptr;
-    if (a.nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
-    if (a.nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
-    if (h.variant == 0)
-        hipLaunchKernelGGL((k_stream<BS, MODE, TV, TX, 0>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
-    else
-        hipLaunchKernelGGL((k_stream<BS, MODE, TV, TX, 1>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
+    if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
+    if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
+    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
     check_launch();
-    if (a.nred > 0) launch_finalize(h, a.nred, a.ra, a.ntiles);
+    if (nred > 0) launch_finalize(h, nred, a.ra, a.ntiles);
 }
 
 int vec_blocks(size_t N) { return (int)((N + kVecElemsPerBlock - 1) / kVecElemsPerBlock); }
 
-// z = M^{-1} p through the two colour sweeps of op
+template <int BS, typename TV, typename TX>
+void sweep_launch(H &h, const DevMat &m, const TV *val, int c, bool dinv_mode, const TX *x, const TX *x_lo, int split,
+                  const TX *rhs, TX *out, const double *dinv, bool use_state) {
+    RedArgs ra = red(h, 0, FIN_NONE);
+    if (dinv_mode)
+        launch_stream<BS, SM_SUB_DINV, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, dinv, nullptr, ra, use_state,
+                                                        nullptr, nullptr, x_lo, split);
+    else
+        launch_stream<BS, SM_SUB, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, nullptr, nullptr, ra, use_state, nullptr,
+                                                   nullptr, x_lo, split);
+}
+
+// z = M^{-1} p through the two colour sweeps of op (first sweep colours
+// ascending into y, second sweep colours descending into z).  Two launches
+// are saved per application:
+//  - when the first sweep has a unit diagonal, its colour-0 rows have no
+//    blocks (nothing precedes colour 0) and y equals p there; that launch is
+//    skipped and later gathers read p for colour-0 columns;
+//  - the last colour has no blocks in the second sweep (nothing follows
+//    it), so its first-sweep launch writes z directly, applying the second
+//    sweep's diagonal inverse in the same epilogue when it has one.
 template <int BS, typename TV, typename TX>
 void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_state) {
     const int nc = h.an.ncolours;
-    RedArgs ra = red(h, 0, FIN_NONE);
+    if (op.s1_dinv && op.s2_dinv) throw std::runtime_error("at most one sweep may carry the diagonal inverse");
+    const bool skip0 = !op.s1_dinv && nc > 1;
+    const TX *lo = skip0 ? p : nullptr;
+    const int split = skip0 ? h.an.colour_ptr[1] : 0;
     prof_begin(h, PK_LOWER);
-    for (int c = 0; c < nc; ++c) {
-        if (op.s1_dinv)
-            launch_stream<BS, SM_SUB_DINV, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, op.dinv, nullptr, 0, ra,
-                                                   use_state);
-        else
-            launch_stream<BS, SM_SUB, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, nullptr, nullptr, 0, ra, use_state);
-    }
+    for (int c = skip0 ? 1 : 0; c < nc - 1; ++c)
+        sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
+    sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
+                             use_state);
     prof_end(h);
     prof_begin(h, PK_UPPER);
-    for (int c = nc - 1; c >= 0; --c) {
-        if (op.s2_dinv)
-            launch_stream<BS, SM_SUB_DINV, TV, TX>(h, *op.S2, op.S2val, c, z, y, z, op.dinv, nullptr, 0, ra,
-                                                   use_state);
-        else
-            launch_stream<BS, SM_SUB, TV, TX>(h, *op.S2, op.S2val, c, z, y, z, nullptr, nullptr, 0, ra, use_state);
-    }
+    for (int c = nc - 2; c >= 0; --c)
+        sweep_launch<BS, TV, TX>(h, *op.S2, op.S2val, c, op.s2_dinv, z, nullptr, 0, (c == 0 && skip0) ? p : y, z,
+                                 op.dinv, use_state);
     prof_end(h);
 }
 
@@ -310,6 +332,8 @@ void init_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
     s.maxit = maxit;
     s.abs_mode = abs_mode ? 1 : 0;
     s.norm_slot = 1;
+    s.host_done = idx == 0 ? h.d_done : nullptr;
+    *(volatile int32_t *)h.h_done = 0;
     *h.h_st = s;
     HIP_CHECK(hipMemcpyAsync(h.st.p + idx, h.h_st, sizeof(SolverState), hipMemcpyHostToDevice, h.stream));
     HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -334,23 +358,26 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     } else {
         if constexpr (std::is_same<TX, double>::value) {
             prof_begin(h, PK_SPMV);
-            launch_stream<BS, SM_SUB, TV, TX>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr, nullptr, 1,
-                                              red(h, 0, FIN_INIT), true, k.r.p, k.p.p);
+            launch_stream<BS, SM_SUB, SR_SELF, TV, TX>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr, nullptr,
+                                                       red(h, 0, FIN_INIT), true, k.r.p, k.p.p);
             prof_end(h);
         } else {
             throw std::runtime_error("non-zero start vector needs FP64 vectors");
         }
     }
-    std::deque<int> pending;
-    int slot = 0;
+    // Iterations are queued ahead of the device; the device stops itself
+    // (every kernel returns at once after the stop) and raises a mapped host
+    // flag.  The host keeps at most `ahead` iterations queued beyond the one
+    // running (bda_set_options' poll_iters, 1..3) and checks the flag each
+    // time it waits.
     bool stop = false;
-    const int poll = std::max(1, h.poll_iters);
+    const int ahead = std::min(3, std::max(1, h.poll_iters));
     for (int it = 1; it <= maxit + 1 && !stop; ++it) {
         h.prof_tag = 2 * (it - 1);
         precond<BS, TV, TX>(h, op, k.p.p, k.y.p, k.phat.p, true);
         prof_begin(h, PK_SPMV);
-        launch_stream<BS, SM_SPMV, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p, 0,
-                                           red(h, 0, FIN_ALPHA), true);
+        launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p,
+                                                  red(h, 0, FIN_ALPHA), true);
         prof_end(h);
         prof_begin(h, PK_VEC);
         hipLaunchKernelGGL((k_update_xs<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
@@ -361,8 +388,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         h.prof_tag = 2 * (it - 1) + 1;
         precond<BS, TV, TX>(h, op, k.s.p, k.y.p, k.shat.p, true);
         prof_begin(h, PK_SPMV);
-        launch_stream<BS, SM_SPMV, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p, 1,
-                                           red(h, 0, FIN_OMEGA), true);
+        launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p,
+                                                    red(h, 0, FIN_OMEGA), true);
         prof_end(h);
         prof_begin(h, PK_VEC);
         hipLaunchKernelGGL((k_update_xr<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
@@ -373,28 +400,14 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                            k.v.p, (const SolverState *)(h.st.p));
         check_launch();
         prof_end(h);
-        if (it % poll == 0 || it == maxit + 1) {
-            HIP_CHECK(hipMemcpyAsync(&h.h_flag[slot], &h.st.p[0].done, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                     h.stream));
-            HIP_CHECK(hipEventRecord(h.poll_ev[slot], h.stream));
-            pending.push_back(slot);
-            slot = (slot + 1) & 3;
-            while (pending.size() > 1) {
-                const int s = pending.front();
-                pending.pop_front();
-                HIP_CHECK(hipEventSynchronize(h.poll_ev[s]));
-                if (h.h_flag[s]) {
-                    stop = true;
-                    break;
-                }
-            }
-            if (h.timeout_s > 0.0) {
-                const double el =
-                    std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-                if (el > h.timeout_s) {
-                    out.aborted = true;
-                    stop = true;
-                }
+        HIP_CHECK(hipEventRecord(h.poll_ev[it & 3], h.stream));
+        if (it > ahead) HIP_CHECK(hipEventSynchronize(h.poll_ev[(it - ahead) & 3]));
+        if (*(volatile int32_t *)h.h_done) stop = true;
+        if (h.timeout_s > 0.0) {
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > h.timeout_s) {
+                out.aborted = true;
+                stop = true;
             }
         }
     }
@@ -472,10 +485,10 @@ void do_analyse(H &h, bool level) {
 
     const size_t bb = (size_t)h.bs * h.bs;
     const int64_t nnzb = an.A.nnzb();
-    h.A64.ensure(nnzb * bb);
+    h.A64.ensure((nnzb + kPadBlocks) * bb);
     h.LU.ensure(nnzb * bb);
-    h.L64.ensure(std::max<int64_t>(an.L.nnzb(), 1) * bb);
-    h.U64.ensure(std::max<int64_t>(an.U.nnzb(), 1) * bb);
+    h.L64.ensure((an.L.nnzb() + kPadBlocks) * bb);
+    h.U64.ensure((an.U.nnzb() + kPadBlocks) * bb);
     h.Dinv.ensure((size_t)h.n * bb);
     h.DinvT.ensure((size_t)h.n * bb);
     h.d_bad.ensure(1);
@@ -531,9 +544,9 @@ void ensure_mixed(H &h) {
     require_factored(h);
     if (h.mixed_gen == h.factor_gen) return;
     const size_t bb = (size_t)BS * BS;
-    h.A32.ensure(h.A.nnzb * bb);
-    h.L32.ensure(std::max<int64_t>(h.L.nnzb, 1) * bb);
-    h.U32.ensure(std::max<int64_t>(h.U.nnzb, 1) * bb);
+    h.A32.ensure((h.A.nnzb + kPadBlocks) * bb);
+    h.L32.ensure((h.L.nnzb + kPadBlocks) * bb);
+    h.U32.ensure((h.U.nnzb + kPadBlocks) * bb);
     gather_blocks<BS, float>(h, h.A.nnzb, nullptr, h.A64.p, h.A32.p, 0);
     gather_blocks<BS, float>(h, h.L.nnzb, nullptr, h.L64.p, h.L32.p, 0);
     gather_blocks<BS, float>(h, h.U.nnzb, nullptr, h.U64.p, h.U32.p, 0);
@@ -546,9 +559,9 @@ void ensure_adjoint(H &h) {
     require_factored(h);
     if (h.adj_gen == h.factor_gen) return;
     const size_t bb = (size_t)BS * BS;
-    h.At64.ensure(std::max<int64_t>(h.At.nnzb, 1) * bb);
-    h.Ut64.ensure(std::max<int64_t>(h.Ut.nnzb, 1) * bb);
-    h.Lt64.ensure(std::max<int64_t>(h.Lt.nnzb, 1) * bb);
+    h.At64.ensure((h.At.nnzb + kPadBlocks) * bb);
+    h.Ut64.ensure((h.Ut.nnzb + kPadBlocks) * bb);
+    h.Lt64.ensure((h.Lt.nnzb + kPadBlocks) * bb);
     gather_blocks<BS, double>(h, h.At.nnzb, h.At.src.p, h.A64.p, h.At64.p, 1);
     gather_blocks<BS, double>(h, h.Ut.nnzb, h.Ut.src.p, h.U64.p, h.Ut64.p, 1);
     gather_blocks<BS, double>(h, h.Lt.nnzb, h.Lt.src.p, h.L64.p, h.Lt64.p, 1);
@@ -630,8 +643,8 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
         // FP64 true residual ro = b - A x
         init_state(h, 1, 0.0, 0, false);
         prof_begin(h, PK_SPMV);
-        launch_stream<BS, SM_SUB, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr, nullptr, 1,
-                                                  red(h, 1, FIN_STORE), true);
+        launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
+                                                           nullptr, red(h, 1, FIN_STORE), true);
         prof_end(h);
         SolverState s1;
         HIP_CHECK(hipMemcpyAsync(h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost, h.stream));
@@ -666,9 +679,13 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
             r.max_iter_reached = 1;
             break;
         }
+        // inner tolerance: inner_tol, or -- adaptive -- only the reduction the
+        // outer target still needs (eta * thr / ||r_k||), whichever is looser
+        double itol = inner_tol;
+        if (h.refine_adaptive) itol = std::min(0.5, std::max(inner_tol, h.refine_eta * thr / nrm));
         RunOut o;
-        if (vec32) mixed_inner<BS, float>(h, h.k32, inner_tol, inner_maxit, o);
-        else mixed_inner<BS, double>(h, h.k64, inner_tol, inner_maxit, o);
+        if (vec32) mixed_inner<BS, float>(h, h.k32, itol, inner_maxit, o);
+        else mixed_inner<BS, double>(h, h.k64, itol, inner_maxit, o);
         r.outer_iterations++;
         r.inner_half_iterations += o.st.h;
         r.iterations += o.st.h / 2.0 + 0.5;
@@ -705,11 +722,11 @@ void spmv_perm(H &h, const double *x, double *y, int transpose) {
     HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * 8, hipMemcpyHostToDevice));
     RedArgs ra = red(h, 1, FIN_NONE);
     if (transpose)
-        launch_stream<BS, SM_SPMV, double, double>(h, h.At, h.At64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr, nullptr,
-                                                   0, ra, false);
+        launch_stream<BS, SM_SPMV, SR_NONE, double, double>(h, h.At, h.At64.p, -1, h.vin.p, nullptr, h.vout.p,
+                                                            nullptr, nullptr, ra, false);
     else
-        launch_stream<BS, SM_SPMV, double, double>(h, h.A, h.A64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr, nullptr, 0,
-                                                   ra, false);
+        launch_stream<BS, SM_SPMV, SR_NONE, double, double>(h, h.A, h.A64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr,
+                                                            nullptr, ra, false);
     HIP_CHECK(hipStreamSynchronize(h.stream));
     HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * 8, hipMemcpyDeviceToHost));
 }
@@ -753,11 +770,11 @@ double time_spmv(H &h, int reps, int fp32) {
     for (int rep = -1; rep < reps; ++rep) {
         if (rep == 0) HIP_CHECK(hipEventRecord(e0, h.stream));
         if (fp32) {
-            launch_stream<BS, SM_SPMV, float, float>(h, h.A, h.A32.p, -1, h.k32.phat.p, nullptr, h.k32.v.p, nullptr,
-                                                     nullptr, 0, ra, false);
+            launch_stream<BS, SM_SPMV, SR_NONE, float, float>(h, h.A, h.A32.p, -1, h.k32.phat.p, nullptr, h.k32.v.p,
+                                                              nullptr, nullptr, ra, false);
         } else {
-            launch_stream<BS, SM_SPMV, double, double>(h, h.A, h.A64.p, -1, h.k64.phat.p, nullptr, h.k64.v.p,
-                                                       nullptr, nullptr, 0, ra, false);
+            launch_stream<BS, SM_SPMV, SR_NONE, double, double>(h, h.A, h.A64.p, -1, h.k64.phat.p, nullptr,
+                                                                h.k64.v.p, nullptr, nullptr, ra, false);
         }
     }
     HIP_CHECK(hipEventRecord(e1, h.stream));
@@ -814,10 +831,10 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         h->st.ensure(2);
         HIP_CHECK(hipHostMalloc((void **)&h->h_st, sizeof(SolverState), hipHostMallocDefault));
-        HIP_CHECK(hipHostMalloc((void **)&h->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc((void **)&h->h_done, sizeof(int32_t), hipHostMallocMapped));
+        HIP_CHECK(hipHostGetDevicePointer((void **)&h->d_done, h->h_done, 0));
         for (auto &e : h->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         h->d_orig.ensure((size_t)nnzb_max * bs * bs);
-        if (const char *v = std::getenv("BDA_STREAM_VARIANT")) h->variant = std::atoi(v) ? 1 : 0;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "ERROR: bda_create: %s\n", e.what());
         bda_destroy(h);
@@ -842,7 +859,7 @@ int bda_destroy(bda_handle h) {
     h->k32.release();
     h->st.release();
     if (h->h_st) (void)hipHostFree(h->h_st);
-    if (h->h_flag) (void)hipHostFree(h->h_flag);
+    if (h->h_done) (void)hipHostFree(h->h_done);
     for (auto &e : h->poll_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &pr : h->ev_pool) {
@@ -942,6 +959,14 @@ int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t pol
         h->timeout_s = timeout_s > 0 ? timeout_s : 0.0;
         h->abs_tol = abs_tol != 0;
         h->poll_iters = poll_iters > 0 ? poll_iters : 1;
+    });
+}
+
+int bda_set_refinement(bda_handle h, int32_t adaptive, double eta) {
+    return guarded(h, [&] {
+        if (!(eta > 0.0 && eta <= 1.0)) throw std::runtime_error("eta must be in (0, 1]");
+        h->refine_adaptive = adaptive != 0;
+        h->refine_eta = eta;
     });
 }
 

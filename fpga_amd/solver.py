@@ -271,6 +271,9 @@ class BicgstabSolver:
                     "bda_solve_mixed_dev")
         return SolveResult.from_c(r)
 
+    def set_refinement(self, adaptive: bool = True, eta: float = 0.5):
+        self._check(lib.bda_set_refinement(self._h, int(adaptive), eta), "bda_set_refinement")
+
     def adjoint_solve(self, w, lam0=None, tol: float = 1e-10, maxit: int = 1000):
         w = _f64(w)
         lam = np.zeros(self.N) if lam0 is None else _f64(lam0).copy()

@@ -139,7 +139,9 @@ int bda_factor(bda_handle h);
 
 /* Host watchdog in seconds (0 = off), counterpart of param0's abort cycles
  * (bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:605-643); abs_tol != 0
- * selects the absolute compare of config_bits bit 1 (solver.vhd:637-641). */
+ * selects the absolute compare of config_bits bit 1 (solver.vhd:637-641);
+ * poll_iters (1..3) is how many iterations the host queues ahead of the
+ * device, which stops itself and raises a host-visible flag. */
 int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t poll_iters);
 
 /* Plain FP64 ILU0-BiCGSTAB, the reference's microprogram
@@ -167,6 +169,12 @@ int bda_solve_mixed(bda_handle h, const double *b, double *x, double tol, int32_
 int bda_solve_mixed_dev(bda_handle h, const double *d_b, double *d_x, double tol,
                         int32_t max_outer, double inner_tol, int32_t inner_maxit,
                         int32_t vec32, bda_result *res);
+/* Refinement policy of the precision form.  adaptive != 0 (default): the
+ * inner tolerance of refinement step k is max(inner_tol, eta*thr/||r_k||),
+ * capped at 0.5, i.e. a step asks only for the reduction the outer target
+ * still needs; adaptive = 0: every inner solve runs to inner_tol.
+ * eta in (0,1], default 0.5. */
+int bda_set_refinement(bda_handle h, int32_t adaptive, double eta);
 
 /* Adjoint: solve A^T lambda = w with the transposed colour-ordered ILU0
  * (U^T sweep over colours ascending, then L^T descending).  lambda holds the

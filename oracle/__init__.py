@@ -82,7 +82,7 @@ def lib():
             "orc_get_factors": [_vp] * 8, "orc_spmv_perm": [_vp, _vp, _vp, _i32],
             "orc_precond_perm": [_vp, _vp, _vp, _i32],
             "orc_solve": [_vp, _vp, _vp, _f64, _i32, _i32, C.POINTER(OrcResult)],
-            "orc_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, C.POINTER(OrcResult)],
+            "orc_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, _i32, _f64, C.POINTER(OrcResult)],
             "orc_adjoint_solve": [_vp, _vp, _vp, _f64, _i32, C.POINTER(OrcResult)],
             "orc_grad": [_vp, _vp, _vp, _vp],
         }.items():
@@ -190,12 +190,13 @@ class OracleSystem:
             raise RuntimeError("orc_solve failed")
         return x, r.as_dict()
 
-    def solve_mixed(self, b, x0=None, tol=1e-6, max_outer=20, inner_tol=1e-4, inner_maxit=1000, vec32=True):
+    def solve_mixed(self, b, x0=None, tol=1e-6, max_outer=20, inner_tol=1e-4, inner_maxit=1000, vec32=True,
+                    adaptive=True, eta=0.5):
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self.N) if x0 is None else np.array(x0, np.float64)
         r = OrcResult()
         if lib().orc_solve_mixed(self._h, _p(b), _p(x), tol, max_outer, inner_tol, inner_maxit, int(vec32),
-                                 C.byref(r)) != 0:
+                                 int(adaptive), eta, C.byref(r)) != 0:
             raise RuntimeError("orc_solve_mixed failed")
         return x, r.as_dict()
 

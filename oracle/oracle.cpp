@@ -652,7 +652,7 @@ static RunState mixed_inner(orc_handle h, const double *rhs, std::vector<double>
 extern "C" {
 
 int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
-                    int32_t inner_maxit, int32_t vec32, orc_result *res) {
+                    int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res) {
     if (!h->factored) return 1;
     if (!h->mixed_ready) {
         h->A32 = convert<float>(h->A);
@@ -691,8 +691,10 @@ int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_
         if (!std::isfinite(nrm)) { r.nan_detected = 1; break; }
         if (outer > 0 && !(nrm > thr)) { r.converged = 1; break; }
         if (outer == max_outer) { r.max_iter_reached = 1; break; }
-        RunState st = vec32 ? mixed_inner<float>(h, ro.data(), xo, inner_tol, inner_maxit)
-                            : mixed_inner<double>(h, ro.data(), xo, inner_tol, inner_maxit);
+        // adaptive: ask only the reduction the outer target still needs
+        const double itol = adaptive ? std::min(0.5, std::max(inner_tol, eta * thr / nrm)) : inner_tol;
+        RunState st = vec32 ? mixed_inner<float>(h, ro.data(), xo, itol, inner_maxit)
+                            : mixed_inner<double>(h, ro.data(), xo, itol, inner_maxit);
         r.outer_iterations++;
         r.inner_half_iterations += st.h;
         r.iterations += st.h / 2.0 + 0.5;

@@ -54,7 +54,7 @@ int orc_precond_perm(orc_handle h, const double *p, double *z, int32_t transpose
 int orc_solve(orc_handle h, const double *b, double *x, double tol, int32_t maxit, int32_t abs_tol,
               orc_result *res);
 int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
-                    int32_t inner_maxit, int32_t vec32, orc_result *res);
+                    int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res);
 int orc_adjoint_solve(orc_handle h, const double *w, double *lam, double tol, int32_t maxit, orc_result *res);
 int orc_grad(orc_handle h, const double *lam, const double *x, double *grad);
 
