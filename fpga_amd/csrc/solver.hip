@@ -828,7 +828,10 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
     h->nnzb_max = nnzb_max;
     try {
         HIP_CHECK(hipGetDevice(&h->device));
-        HIP_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        // a blocking stream: it orders itself after work the caller queued on
+        // the legacy default stream (torch's default), so device inputs
+        // produced there are complete before the handle reads them
+        HIP_CHECK(hipStreamCreate(&h->stream));
         h->st.ensure(2);
         HIP_CHECK(hipHostMalloc((void **)&h->h_st, sizeof(SolverState), hipHostMallocDefault));
         HIP_CHECK(hipHostMalloc((void **)&h->h_done, sizeof(int32_t), hipHostMallocMapped));

@@ -21,6 +21,9 @@
  *     owned by the handle; "_dev" entry points take device pointers
  *     (hipMalloc'ed, e.g. a torch tensor's data_ptr) and are synchronous;
  *   - one HIP stream per handle, every call blocks until its work is done;
+ *     the stream is a blocking stream, so "_dev" inputs written by work on
+ *     the legacy default stream are complete before the handle reads them;
+ *     inputs written on other streams must be synchronised by the caller;
  *   - a handle must not be used from two threads at once.
  *
  * Vectors passed to bda_solve*, bda_adjoint_solve* and bda_grad_values are in
