@@ -42,6 +42,8 @@ def parse():
                     help="kernels: only repeated SpMV launches (FP64 and FP32 values) and a "
                          "reference copy of known size, for rocprofv3 counter runs")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="also time the adjoint backward (A^T lambda = w to 1e-10, then -lambda x^T)")
     return ap.parse_args()
 
 
@@ -129,14 +131,50 @@ def main():
     t_mixed = (time.perf_counter() - t0) / args.steps
     errmx = (torch.linalg.vector_norm(x - x_ref).item() / nref)
 
+    adj = None
+    if args.adjoint:
+        w = torch.from_numpy(np.random.default_rng(0).standard_normal(s.N)).to(dev)
+        lam = torch.zeros_like(w)
+        grad = torch.empty((s.nnzb, s.bs, s.bs), dtype=torch.float64, device=dev)
+        g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, maxit)   # builds the transposed operators
+        times, ra = [], None
+        for _ in range(args.steps):
+            lam.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ra = g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, maxit)
+            g.grad_values_dev(lam.data_ptr(), x_ref.data_ptr(), grad.data_ptr())
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        adj = {"time_s": float(np.mean(times)), "tol": 1e-10, "iterations": ra.iterations,
+               "converged": ra.converged, "rel_residual": ra.rel_residual}
+        adj_pending = True
+    else:
+        adj_pending = False
+
     spmv_bytes = g.algorithmic_bytes(fa.KIND_SPMV)
-    spmv_ms = ms_spmv / max(n_spmv, 1)
+    # dominant kernel: the solve's SpMV (v = A phat with the fused v.rt sums),
+    # 20 back-to-back launches between two HIP events on the solver's stream
+    spmv_ms = g.time_spmv(20, False)
     spmv_gbs = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    spmv32_ms = g.time_spmv(20, True)
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_spmv_{args.workload}.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text())
+        traffic = (t["fetch_kib"] * t["fetch_factor"] + t["write_kib"] * t["write_factor"]) * 1024.0
     low_bytes, up_bytes = g.algorithmic_bytes(fa.KIND_LOWER), g.algorithmic_bytes(fa.KIND_UPPER)
     iters64 = r64.iterations
     # algorithmic bytes per FP64 iteration: 2 SpMV + 2 (L + U) + 22 N vector passes
     it_bytes = 2 * spmv_bytes + 2 * (low_bytes + up_bytes) + 22 * s.N * 8
     solve_gbs = it_bytes * iters64 / t_fp64 / 1e9
+    if adj_pending:
+        # same per-iteration traffic (A^T, U^T, L^T have the blocks of A, U, L) plus the gradient pass
+        grad_bytes = s.nnzb * s.bs * s.bs * 8 + s.nnzb * 12 + 2 * s.N * 8
+        adj["GBs_algorithmic"] = (it_bytes * adj["iterations"] + grad_bytes) / adj["time_s"] / 1e9
+    mixed_it_bytes = (2 * g.algorithmic_bytes(fa.KIND_SPMV, True)
+                      + 2 * (g.algorithmic_bytes(fa.KIND_LOWER, True) + g.algorithmic_bytes(fa.KIND_UPPER, True))
+                      + 22 * s.N * 4)
 
     cpu = None
     if not args.no_cpu:
@@ -176,13 +214,19 @@ def main():
                  "solve_GBs_algorithmic": solve_gbs},
         "mixed": {"time_s": t_mixed, "outer": rmx.outer_iterations, "inner_iterations": rmx.iterations,
                   "rel_residual": rmx.rel_residual, "x_rel_err_vs_1e-12": errmx,
-                  "speedup_vs_fp64": t_fp64 / t_mixed},
-        "kernels_ms_avg": {"spmv": spmv_ms, "lower_sweep": ms_low / max(n_low, 1),
-                           "upper_sweep": ms_up / max(n_up, 1), "vector_updates_per_iter": ms_vec / max(n_vec, 1)},
+                  "speedup_vs_fp64": t_fp64 / t_mixed,
+                  "inner_GBs_algorithmic_approx": mixed_it_bytes * rmx.iterations / t_mixed / 1e9},
+        "kernels_ms_avg": {"spmv": spmv_ms, "spmv_fp32": spmv32_ms,
+                           "in_solve_event_pairs": {"spmv": ms_spmv / max(n_spmv, 1),
+                                                    "lower_sweep": ms_low / max(n_low, 1),
+                                                    "upper_sweep": ms_up / max(n_up, 1),
+                                                    "vector_updates_per_half_iter": ms_vec / max(n_vec, 1)}},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "k_stream<SPMV> FP64",
+                     "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stream<3, SM_SPMV, SR_W0, double, double>",
                      "algorithmic_bytes_per_launch": spmv_bytes},
         "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac},
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

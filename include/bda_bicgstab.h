@@ -210,8 +210,10 @@ int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *
  * updates.  Counts and summed milliseconds since the last reset. */
 int bda_set_profiling(bda_handle h, int32_t on);
 int bda_get_kernel_stats(bda_handle h, int32_t kind, int64_t *launches, double *total_ms);
-/* reps back-to-back FP64 (or FP32-value, fp32 != 0) SpMV launches on
- * device-resident vectors; average launch time in ms from HIP events. */
+/* reps back-to-back launches of the solve's SpMV kernel (v = A phat with
+ * the fused v.rt partial sums; FP64, or FP32 values and vectors with
+ * fp32 != 0) on device-resident vectors; average launch time in ms from two
+ * HIP events on the handle's stream. */
 int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
 /* Algorithmic bytes of one launch of the kernel classes above (FP64 form). */
 int bda_algorithmic_bytes(bda_handle h, int32_t kind, int32_t fp32, double *bytes);
