@@ -67,6 +67,8 @@ SIGNATURES = {
     "bda_grid_copy": (i32, [vp, vp, vp, vp, vp]),
     "bda_grid_destroy": (i32, [vp]),
     "bda_colour_pattern": (i32, [i32, vp, vp, i32, vp, vp, vp, P_i32]),
+    "bda_encode_status": (i32, [C.POINTER(BdaResult), vp]),
+    "bda_encode_query": (i32, [C.POINTER(BdaLimits), i32, i64, i32, vp]),
     "bda_create": (i32, [i32, i32, i32, C.POINTER(vp)]),
     "bda_destroy": (i32, [vp]),
     "bda_last_error": (C.c_char_p, [vp]),

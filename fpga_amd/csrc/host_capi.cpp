@@ -1,5 +1,6 @@
 // extern "C" access to the host-only parts: the synthetic grid generator and
 // the colouring / colour-ordered permutation (no device needed).
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -94,3 +95,54 @@ int bda_colour_pattern(int32_t n, const int32_t *rowptr, const int32_t *colidx, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Status and limits lines in the layout of the reference kernel's debug
+// port, so that its host decoder reads them unchanged.
+//   status line 0 (bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:75-82):
+//     bit 0 aborted, bit 1 no results, bits 95..64 cycles, bits 511..488 "BDA"
+//   status line 1 (decoded at bicgstab_utils.cpp:138-259): word 0 overflow
+//     flags, word 2 bits 63..48 solver state, word 3 bits 31..16 line count
+//     and 47..32 half-iteration count, words 4..7 the four norms
+//   limits line (wrapper vhd:51-73, decoded at bicgstab_utils.cpp:282-369)
+namespace {
+constexpr uint64_t kSignature = 0x414442ULL;   // "BDA"
+constexpr double kKernelClockHz = 250e6;      // the reference kernel clock (ooc.xdc:8)
+uint64_t dbits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return u;
+}
+}  // namespace
+
+int bda_encode_status(const bda_result *r, uint64_t line[16]) {
+    if (!r || !line) return 2;
+    std::memset(line, 0, 16 * sizeof(uint64_t));
+    line[0] = (r->aborted ? 1ULL : 0ULL) | (r->no_results ? 2ULL : 0ULL);
+    double cyc = r->solve_ms * 1e-3 * kKernelClockHz;
+    if (!(cyc >= 0.0)) cyc = 0.0;
+    line[1] = cyc >= 4294967295.0 ? 0xFFFFFFFFULL : (uint64_t)std::llround(cyc);
+    line[7] = kSignature << 40;
+    line[8] = 0;                                   // no overflow / underflow flags
+    line[10] = 0xCULL << 48;                       // solver state write_debug (solver_pkg.vhd:69)
+    line[11] = (1ULL << 16) | ((uint64_t)(uint16_t)r->half_iterations << 32);
+    for (int i = 0; i < 4; ++i) line[12 + i] = dbits(r->norms[i]);
+    return 0;
+}
+
+int bda_encode_query(const bda_limits *lim, int32_t n, int64_t nnzb_max, int32_t bs, uint64_t line[8]) {
+    if (!lim || !line || n <= 0 || nnzb_max < 0 || (bs != 3 && bs != 4)) return 2;
+    std::memset(line, 0, 8 * sizeof(uint64_t));
+    const uint64_t rows = (uint64_t)n * bs;
+    auto sat32 = [](uint64_t v) { return v > 0xFFFFFFFFULL ? 0xFFFFFFFFULL : v; };
+    line[0] = sat32(rows) | (sat32(rows) << 32);                 // x vector elements, max rows
+    line[1] = sat32(rows) | (sat32((uint64_t)n) << 32);          // max gathered columns, max colours
+    const uint64_t nnz_row = (uint64_t)lim->max_blocks_per_row * bs;
+    line[2] = (nnz_row > 0xFFFF ? 0xFFFFULL : nnz_row) | (sat32((uint64_t)nnzb_max * bs * bs) << 16);
+    line[6] = 0ULL                                               // no URAM
+              | (1ULL << 1)                                      // ILU0 results written
+              | ((uint64_t)512 << 16)                            // data path width in bits
+              | ((uint64_t)(lim->wavefront_size & 0xFF) << 56);  // parallel multipliers per wave
+    line[7] = 1ULL | (1ULL << 4) | (kSignature << 40);           // one read / one write stream
+    return 0;
+}

@@ -105,6 +105,21 @@ int bda_grid_destroy(bda_grid g);
 int bda_colour_pattern(int32_t n, const int32_t *rowptr, const int32_t *colidx, int32_t level_scheduling,
                        int32_t *colour, int32_t *perm, int32_t *colour_ptr, int32_t *num_colours);
 
+/* ---- reference-compatible status and limits lines (host only) ---------
+ * 16 words = the kernel's status line 0 and its first debug line, laid out
+ * as the reference wrapper writes them (bicgstab_solver_wrapper_2r_3r3w_
+ * simple-ddr_v1.vhd:51-82) and its host decodes them
+ * (decode_debuginfo_bicgstab, bicgstab_utils.cpp:90-280): abort and
+ * "no results" bits, cycles in units of the reference's 250 MHz kernel
+ * clock, "BDA" signature, half-iteration count, the four norms.
+ * Breakdown and NaN have no field there; they are reported in bda_result. */
+int bda_encode_status(const bda_result *res, uint64_t line[16]);
+/* 8 words = the limits line of a kernel query (fpga_kernel_query,
+ * fpga_functions_bicgstab.cpp:1332-1452; decode_debuginfo_bicgstab_query,
+ * bicgstab_utils.cpp:282-369), filled with this solver's capacities for a
+ * handle of n block rows, nnzb_max blocks and block size bs. */
+int bda_encode_query(const bda_limits *limits, int32_t n, int64_t nnzb_max, int32_t bs, uint64_t line[8]);
+
 /* ---- handle lifecycle -------------------------------------------------- */
 /* Allocate for a system of n block rows and at most nnzb_max blocks of
  * size bs x bs (bs in {3,4}).  Replaces fpga_setup_host_datamem +

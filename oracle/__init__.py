@@ -63,6 +63,37 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+REF_DIR = HERE / "_ref"
+REF_STATUS_LIB = REF_DIR / "libbda_status_ref.so"
+REFERENCE_SRC = Path("/root/reference/linearalgebra/ilu0bicgstab/xilinx/src/sda_app/common")
+
+
+def build_ref(verbose: bool = False):
+    """Build the reference's status decoder into oracle/_ref (needs the
+    reference sources, i.e. only where /root/reference exists)."""
+    if not REFERENCE_SRC.exists():
+        return None
+    res = subprocess.run(["make", "-s", "-f", str(HERE / "ref.mk"), f"OUT={REF_DIR}", f"REF={REFERENCE_SRC}"],
+                         capture_output=True, text=True, cwd=str(HERE.parent))
+    if res.returncode != 0:
+        raise RuntimeError(f"reference decoder build failed:\n{res.stderr}")
+    if verbose:
+        print(f"built {REF_STATUS_LIB}")
+    return REF_STATUS_LIB
+
+
+def ref_status_decoder():
+    """ctypes handles of the reference's decode_debuginfo_bicgstab and
+    decode_debuginfo_bicgstab_query (C++ linkage, called by mangled name)."""
+    if not REF_STATUS_LIB.exists():
+        return None
+    L = C.CDLL(str(REF_STATUS_LIB))
+    dec = getattr(L, "_Z25decode_debuginfo_bicgstabbbPmjjjPjS0_PdPhPbS3_S3_S3_S3_S3_")
+    q = getattr(L, "_Z31decode_debuginfo_bicgstab_querybPmjjPjS0_S0_S0_PtS0_PbS2_S1_PhS3_S3_S3_S3_S3_S1_S1_")
+    dec.restype = q.restype = _i32
+    return dec, q
+
+
 _lib = None
 
 
