@@ -82,17 +82,17 @@ int colour_rows(const Bcsr &m, bool level_scheduling, std::vector<int32_t> &colo
     return ncol;
 }
 
-void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) {
-    const int cap = tile_cap(bs);
+void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, int cap,
+                 std::vector<int32_t> &tile_row, std::vector<int32_t> &colour_tile) {
     const int ncol = (int)colour_ptr.size() - 1;
-    p.tile_row.clear();
-    p.colour_tile.assign(ncol + 1, 0);
+    tile_row.clear();
+    colour_tile.assign(ncol + 1, 0);
     for (int c = 0; c < ncol; ++c) {
-        p.colour_tile[c] = (int)p.tile_row.size();
+        colour_tile[c] = (int)tile_row.size();
         int r = colour_ptr[c];
         const int end = colour_ptr[c + 1];
         while (r < end) {
-            p.tile_row.push_back(r);
+            tile_row.push_back(r);
             int rows = 0, blocks = 0;
             while (r < end && rows < kTileRows) {
                 const int nb = p.rowptr[r + 1] - p.rowptr[r];
@@ -107,8 +107,13 @@ void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) 
             }
         }
     }
-    p.colour_tile[ncol] = (int)p.tile_row.size();
-    p.tile_row.push_back(colour_ptr[ncol]);
+    colour_tile[ncol] = (int)tile_row.size();
+    tile_row.push_back(colour_ptr[ncol]);
+}
+
+void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) {
+    build_tiles(p, colour_ptr, tile_cap(bs, false), p.tile_row, p.colour_tile);
+    build_tiles(p, colour_ptr, tile_cap(bs, true), p.tile_row32, p.colour_tile32);
 }
 
 void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {

@@ -296,9 +296,11 @@ __device__ inline float load_stream(const float *p) { return *p; }
 constexpr int kPadBlocks = 8;
 
 // Tile capacity in blocks; the host tiler uses the same numbers.
-template <int BS> struct TileCap;
-template <> struct TileCap<3> { static constexpr int value = 168; };  // 8 workgroup steps of 21 blocks
-template <> struct TileCap<4> { static constexpr int value = 128; };  // 8 workgroup steps of 16 blocks
+template <int BS, typename TV> struct TileCap;
+template <> struct TileCap<3, double> { static constexpr int value = 168; };  // 8 workgroup steps of 21 blocks
+template <> struct TileCap<4, double> { static constexpr int value = 128; };  // 8 workgroup steps of 16 blocks
+template <> struct TileCap<3, float> { static constexpr int value = 336; };   // FP32 values: 16 steps
+template <> struct TileCap<4, float> { static constexpr int value = 256; };
 
 // Matrix values and column indices are read once per launch; the FP64
 // values and the column indices use non-temporal loads, so that they evict
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
-    constexpr int CAP = TileCap<BS>::value;
+    constexpr int CAP = TileCap<BS, TV>::value;
     constexpr int BPW = 64 / BB;           // blocks per wave step (7 or 4)
     constexpr int VPW = BPW * BB;          // lanes carrying a value (63 or 64)
     constexpr int S = CAP / (NW * BPW);    // value slots per lane

@@ -58,31 +58,41 @@ struct DevMat {
     int64_t nnzb = 0;
     int ntiles = 0;
     DBuf<int32_t> rowptr, colidx, src;
-    DBuf<int2> tile;  // {first row, first block} per tile, plus the end
-    std::vector<int32_t> colour_tile;
+    DBuf<int2> tile;    // {first row, first block} per tile, plus the end
+    DBuf<int2> tile32;  // tiles of the FP32-value kernels
+    std::vector<int32_t> colour_tile, colour_tile32;
+    int ntiles32 = 0;
     void upload(const MatPattern &p, hipStream_t s) {
         n = p.n;
         nnzb = p.nnzb();
         ntiles = p.ntiles();
         colour_tile = p.colour_tile;
+        colour_tile32 = p.colour_tile32;
+        ntiles32 = (int)p.tile_row32.size() - 1;
         rowptr.ensure(p.rowptr.size());
         colidx.ensure(p.colidx.size() + kPadBlocks);
         src.ensure(std::max<size_t>(p.src.size(), 1));
         tile.ensure(p.tile_row.size());
         std::vector<int2> desc(p.tile_row.size());
         for (size_t i = 0; i < desc.size(); ++i) desc[i] = make_int2(p.tile_row[i], p.rowptr[p.tile_row[i]]);
+        tile32.ensure(p.tile_row32.size());
+        std::vector<int2> desc32(p.tile_row32.size());
+        for (size_t i = 0; i < desc32.size(); ++i)
+            desc32[i] = make_int2(p.tile_row32[i], p.rowptr[p.tile_row32[i]]);
         HIP_CHECK(hipMemcpyAsync(rowptr.p, p.rowptr.data(), p.rowptr.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.colidx.empty())
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipMemcpyAsync(tile.p, desc.data(), desc.size() * sizeof(int2), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(tile32.p, desc32.data(), desc32.size() * sizeof(int2), hipMemcpyHostToDevice, s));
         HIP_CHECK(hipStreamSynchronize(s));
     }
     void release() {
         rowptr.release();
         colidx.release();
         tile.release();
+        tile32.release();
         src.release();
     }
 };
@@ -178,7 +188,9 @@ struct bda_handle_s {
 namespace bda {
 namespace {
 
-static_assert(TileCap<3>::value == 168 && TileCap<4>::value == 128, "device tile capacity must match tile_cap()");
+static_assert(TileCap<3, double>::value == 168 && TileCap<4, double>::value == 128 &&
+                  TileCap<3, float>::value == 336 && TileCap<4, float>::value == 256,
+              "device tile capacity must match tile_cap()");
 
 using H = bda_handle_s;
 
@@ -245,13 +257,15 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
                    TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
-    a.tile = m.tile.p;
+    constexpr bool f32 = std::is_same<TV, float>::value;
+    const std::vector<int32_t> &ct = f32 ? m.colour_tile32 : m.colour_tile;
+    a.tile = f32 ? m.tile32.p : m.tile.p;
     if (colour < 0) {
         a.tile0 = 0;
-        a.ntiles = m.ntiles;
+        a.ntiles = f32 ? m.ntiles32 : m.ntiles;
     } else {
-        a.tile0 = m.colour_tile[colour];
-        a.ntiles = m.colour_tile[colour + 1] - m.colour_tile[colour];
+        a.tile0 = ct[colour];
+        a.ntiles = ct[colour + 1] - ct[colour];
     }
     if (a.ntiles <= 0) return;
     a.rowptr = m.rowptr.p;
@@ -280,9 +294,10 @@ template <int BS, typename TV, typename TX>
 void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, RedArgs ra) {
     StreamArgs<TV, TX> a;
     std::memset(&a, 0, sizeof(a));
-    a.tile = h.A.tile.p;
+    constexpr bool f32 = std::is_same<TV, float>::value;
+    a.tile = f32 ? h.A.tile32.p : h.A.tile.p;
     a.tile0 = 0;
-    a.ntiles = h.A.ntiles;
+    a.ntiles = f32 ? h.A.ntiles32 : h.A.ntiles;
     a.rowptr = h.A.rowptr.p;
     a.colidx = h.A.colidx.p;
     a.val = val;
