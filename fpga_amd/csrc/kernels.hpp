@@ -286,6 +286,14 @@ __device__ inline int xcd_tile(int b, int G) {
     return x * q + (x < rem ? x : rem) + i;
 }
 
+// Value of lane i + 1 (lane 63 receives 0): a DPP wavefront shift, a VALU
+// operation, where __shfl_down would go through the LDS crossbar.
+__device__ inline double lane_down1(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130 /* wave_shl:1 */, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // FP64 matrix values are loaded non-temporally; FP32 values are not (measured
 // on MI355X: the hint helps the FP64 stream, costs the FP32 one a few %).
 __device__ inline double load_stream(const double *p) { return __builtin_nontemporal_load(p); }
@@ -407,19 +415,22 @@ __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     for (int j = 0; j < XG; ++j) sx[t + j * NT] = xg[j];
     __syncthreads();
 
+    // all LDS reads first (clamped, unconditional), then the products
+    TX xs[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) xs[u] = sx[min((u * NW + wave) * BPW + lb, CAP - 1) * BS + ec];
 #pragma unroll
     for (int u = 0; u < S; ++u) {
         const int bl = (u * NW + wave) * BPW + lb;
         const bool ok = active_lane && bl < nblk;
-        const TX xv = ok ? sx[bl * BS + ec] : (TX)0;
-        const double p = prod<TV, TX>(ok ? av[u] : (TV)0, xv);
+        const double p = prod<TV, TX>(ok ? av[u] : (TV)0, ok ? xs[u] : (TX)0);
         double sm = p;
-        const double p1 = __shfl_down(p, 1);
-        const double p2 = __shfl_down(p, 2);
+        const double p1 = lane_down1(p);
+        const double p2 = lane_down1(p1);
         sm = sm + p1;
         sm = sm + p2;
         if constexpr (BS == 4) {
-            const double p3 = __shfl_down(p, 3);
+            const double p3 = lane_down1(p2);
             sm = sm + p3;
         }
         if (ok && ec == 0) part[bl * BS + er] = sm;

@@ -132,6 +132,7 @@ struct bda_handle_s {
     int bs = 0, n = 0;
     int64_t nnzb_max = 0;
     int device = 0;
+    int cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -250,6 +251,15 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
     check_launch();
 }
 
+// Launches the streaming kernel, one workgroup per tile; returns the number
+// of reduction partials written.
+template <int BS, int MODE, int RED, typename TV, typename TX>
+int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
+    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    check_launch();
+    return a.ntiles;
+}
+
 // --- streaming kernel launch over the tiles of one colour (or all) ---------
 template <int BS, int MODE, int RED, typename TV, typename TX>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
@@ -285,9 +295,8 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
     if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
-    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
-    check_launch();
-    if (nred > 0) launch_finalize(h, nred, a.ra, a.ntiles);
+    const int nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+    if (nred > 0) launch_finalize(h, nred, a.ra, nparts);
 }
 
 template <int BS, typename TV, typename TX>
@@ -308,8 +317,7 @@ void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, 
     a.w0 = w0;
     a.ra = ra;
     if (a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
-    hipLaunchKernelGGL((k_stream<BS, SM_SPMV, SR_W0, TV, TX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
-    check_launch();
+    stream_dispatch<BS, SM_SPMV, SR_W0, TV, TX>(h, a);
 }
 
 int vec_blocks(size_t N) { return (int)((N + kVecElemsPerBlock - 1) / kVecElemsPerBlock); }
@@ -866,6 +874,7 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
     h->nnzb_max = nnzb_max;
     try {
         HIP_CHECK(hipGetDevice(&h->device));
+        HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
         // a blocking stream: it orders itself after work the caller queued on
         // the legacy default stream (torch's default), so device inputs
         // produced there are complete before the handle reads them
