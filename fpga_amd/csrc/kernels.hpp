@@ -41,7 +41,17 @@ struct SolverState {
     int32_t done;
     int32_t converged, no_results, max_iter, breakdown, nan_flag;
     int32_t last_half;      // 1: s is the latest residual, 2: r is
-    int32_t *host_done;     // mapped host memory: set to 1 when done is set
+    int32_t run_id;         // written to host_done when the run stops
+    int32_t *host_done;     // mapped host memory the host polls
+    // outer refinement of the precision form (used on the outer state)
+    double nrm0;            // FP64 residual norm of the start vector
+    double itol;            // inner tolerance for the next inner run
+    double inner_tol, eta;  // adaptive rule: min(0.5, max(inner_tol, eta thr / ||r_k||))
+    double iters;           // sum of the inner runs' iteration counts
+    int32_t outer;          // inner runs completed
+    int32_t max_outer;
+    int32_t adaptive;
+    int32_t inner_h;        // sum of the inner runs' half iterations
 };
 
 enum Fin : int32_t {
@@ -52,6 +62,7 @@ enum Fin : int32_t {
     FIN_HALF1 = 4,   // ||s||, compare, final-iteration exit
     FIN_OMEGA = 5,   // omega = (t.s) / (t.t)
     FIN_HALF2 = 6,   // ||r||, compare, rho_new = rt.r, beta
+    FIN_REFINE = 7,  // outer refinement: ||b - A x||, stop tests, next inner tolerance
 };
 
 // Threshold below which the initial residual counts as already solved in
@@ -75,9 +86,11 @@ __device__ inline void push_norm(SolverState *st, double nrm) {
 
 __device__ inline bool finite(double v) { return !__builtin_isnan(v) && !__builtin_isinf(v); }
 
+// The flag receives the run's id, so the host never has to clear it between
+// runs (a clear could race a late write of the previous run).
 __device__ inline void signal_done(SolverState *st) {
     st->done = 1;
-    if (st->host_done) __hip_atomic_store(st->host_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (st->host_done) __hip_atomic_store(st->host_done, st->run_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Scalar epilogues; run by one thread after a grid-wide reduction.
@@ -147,9 +160,67 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         }
         break;
     }
+    case FIN_REFINE: {  // one outer step of the precision form
+        const double nrm = sqrt(t0);
+        if (st->outer == 0) {
+            st->nrm0 = nrm;
+            st->norms[0] = nrm;
+            const double precision = st->abs_mode ? st->tol : kAlreadySolved;
+            if (!(nrm > precision)) {
+                st->no_results = 1;
+                if (__builtin_isnan(nrm)) st->nan_flag = 1;
+                signal_done(st);
+                break;
+            }
+            st->thr = st->abs_mode ? st->tol : st->tol * nrm;
+            st->last_norm = nrm;
+        } else {
+            push_norm(st, nrm);
+        }
+        if (!finite(nrm)) { st->nan_flag = 1; signal_done(st); }
+        else if (st->outer > 0 && !(nrm > st->thr)) { st->converged = 1; signal_done(st); }
+        else if (st->outer == st->max_outer) { st->max_iter = 1; signal_done(st); }
+        else if (st->adaptive) {
+            const double want = st->eta * st->thr / nrm;
+            const double m = st->inner_tol < want ? want : st->inner_tol;
+            st->itol = m < 0.5 ? m : 0.5;
+        } else {
+            st->itol = st->inner_tol;
+        }
+        break;
+    }
     default:
         break;
     }
+}
+
+// Solver state set-up on the stream (no host copy, no synchronisation).
+__global__ void k_set_state(SolverState *st, SolverState v) { *st = v; }
+
+// Start of an inner run of the precision form: tolerance from the outer
+// state; an outer state that has already stopped stops the inner run.
+__global__ void k_inner_start(SolverState *in, const SolverState *out, SolverState v) {
+    *in = v;
+    if (out->done) {
+        in->no_results = 1;
+        signal_done(in);
+    } else {
+        in->tol = out->itol;
+    }
+}
+
+// End of an inner run: counters and flags into the outer state.  An inner
+// run whose right-hand side was already zero ends the refinement.
+__global__ void k_refine_step(SolverState *out, const SolverState *in) {
+    if (out->done) return;
+    out->outer += 1;
+    out->inner_h += in->h;
+    out->iters += in->h / 2.0 + 0.5;
+    if (in->breakdown || in->nan_flag) {
+        out->breakdown = in->breakdown;
+        out->nan_flag = in->nan_flag;
+    }
+    if (in->no_results) signal_done(out);
 }
 
 __device__ inline double wave_sum(double v) {
@@ -319,6 +390,14 @@ template <> struct TileCap<4, float> { static constexpr int value = 256; };
 // Reductions fused into the epilogue: bit 0 = sum out*w0, bit 1 = sum out*out.
 enum StreamRed : int { SR_NONE = 0, SR_W0 = 1, SR_SELF = 2, SR_BOTH = 3 };
 
+// Minimum waves per SIMD requested from the register allocator: 8 for the
+// FP32 3x3 product kernel, whose natural allocation lands just above 64
+// VGPRs (7 waves); 1 (no request) elsewhere, where 8 would cause spills.
+template <int BS, int MODE, typename TV, typename TX> struct StreamWaves {
+    static constexpr int value =
+        (std::is_same<TV, float>::value && std::is_same<TX, float>::value && BS == 3 && MODE == SM_SPMV) ? 8 : 1;
+};
+
 // Phase 1 has no loop: every lane owns a fixed set of S value slots of the
 // tile (S = CAP / blocks per workgroup step) and issues all their loads at
 // once.  The x entries a tile needs are gathered cooperatively into LDS,
@@ -333,7 +412,7 @@ enum StreamRed : int { SR_NONE = 0, SR_W0 = 1, SR_SELF = 2, SR_BOTH = 3 };
 // for the value stream issued after them.  A tile costs three dependent
 // memory round trips: descriptor; columns/values/row operands; x gathers.
 template <int BS, int MODE, int RED, typename TV, typename TX>
-__global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
+__global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamWaves<BS, MODE, TV, TX>::value))) void k_stream(StreamArgs<TV, TX> a) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup

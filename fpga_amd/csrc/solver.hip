@@ -161,8 +161,9 @@ struct bda_handle_s {
     int32_t partial_cap = 0;
     bda::DBuf<bda::SolverState> st;
     bda::SolverState *h_st = nullptr;  // pinned
-    int32_t *h_done = nullptr;          // mapped host flag the device sets when a solve stops
-    int32_t *d_done = nullptr;          // its device address
+    int32_t *h_done = nullptr;          // mapped host flags, one per state: the id of the run that stopped
+    int32_t *d_done = nullptr;          // their device address
+    int32_t run_seq = 0;                // last run id handed out
     hipEvent_t poll_ev[4] = {};
 
     // options
@@ -369,28 +370,52 @@ struct RunOut {
     bool aborted = false;
 };
 
-void init_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
+// Fresh state for a run on state slot idx.  The state is written by a kernel
+// on the stream, so no host synchronisation is needed; the run's id goes to
+// the slot's mapped flag when the run stops.
+SolverState new_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
     SolverState s;
     std::memset(&s, 0, sizeof(s));
     s.tol = tol;
     s.maxit = maxit;
     s.abs_mode = abs_mode ? 1 : 0;
     s.norm_slot = 1;
-    s.host_done = idx == 0 ? h.d_done : nullptr;
-    *(volatile int32_t *)h.h_done = 0;
-    *h.h_st = s;
-    HIP_CHECK(hipMemcpyAsync(h.st.p + idx, h.h_st, sizeof(SolverState), hipMemcpyHostToDevice, h.stream));
-    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.run_seq = h.run_seq == INT32_MAX ? 1 : h.run_seq + 1;
+    s.run_id = h.run_seq;
+    s.host_done = h.d_done + idx;
+    return s;
+}
+
+bool run_stopped(const H &h, int idx, int32_t run_id) {
+    return __atomic_load_n(h.h_done + idx, __ATOMIC_ACQUIRE) == run_id;
+}
+
+int32_t init_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
+    const SolverState s = new_state(h, idx, tol, maxit, abs_mode);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, h.stream, h.st.p + idx, s);
+    check_launch();
+    return s.run_id;
 }
 
 // One BiCGSTAB run following the reference microprogram (solver_pkg.vhd:120-141).
 // zero_x0: x0 = 0 and the initial residual is the right-hand side itself.
+// outer != nullptr: an inner run of the precision form; its tolerance comes
+// from the outer state on the device, and the run ends without reading its
+// state back (the outer state collects the counters).
 template <int BS, typename TV, typename TX>
 RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs, bool zero_x0, double tol,
-                    int maxit, bool abs_mode) {
+                    int maxit, bool abs_mode, const SolverState *outer = nullptr) {
     const size_t N = h.N();
     const int vb = vec_blocks(N);
-    init_state(h, 0, tol, maxit, abs_mode);
+    int32_t run_id;
+    if (outer) {
+        const SolverState s = new_state(h, 0, 0.0, maxit, false);
+        run_id = s.run_id;
+        hipLaunchKernelGGL(k_inner_start, dim3(1), dim3(1), 0, h.stream, h.st.p, outer, s);
+        check_launch();
+    } else {
+        run_id = init_state(h, 0, tol, maxit, abs_mode);
+    }
     RunOut out;
     const auto t0 = std::chrono::steady_clock::now();
     h.prof_tag = -1;
@@ -446,7 +471,7 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         prof_end(h);
         HIP_CHECK(hipEventRecord(h.poll_ev[it & 3], h.stream));
         if (it > ahead) HIP_CHECK(hipEventSynchronize(h.poll_ev[(it - ahead) & 3]));
-        if (*(volatile int32_t *)h.h_done) stop = true;
+        if (run_stopped(h, 0, run_id)) stop = true;
         if (h.timeout_s > 0.0) {
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (el > h.timeout_s) {
@@ -454,6 +479,10 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                 stop = true;
             }
         }
+    }
+    if (outer) {
+        h.prof_tag = -1;
+        return out;
     }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -656,14 +685,22 @@ void solve_dev(H &h, const double *d_b, double *d_x, double tol, int maxit, bda_
     h.last_kind = 0;
 }
 
-// precision form: FP64 refinement around FP32-value inner solves
+// precision form: FP64 refinement around FP32-value inner solves.  The outer
+// loop runs on the device: the FP64 residual's epilogue (FIN_REFINE) makes
+// the stop decisions and sets the next inner tolerance, k_inner_start and
+// k_refine_step link the inner runs to the outer state.  The host waits once
+// per outer step, after the residual, to learn whether another inner run is
+// needed.
 template <int BS, typename TX>
-void mixed_inner(H &h, Krylov<TX> &k, double inner_tol, int inner_maxit, RunOut &o) {
+bool mixed_inner(H &h, Krylov<TX> &k, int inner_maxit) {
     OpSet<float> op = op_fwd32(h);
-    o = run_bicgstab<BS, float, TX>(h, op, k, h.ro.p, true, inner_tol, inner_maxit, false);
+    const RunOut o = run_bicgstab<BS, float, TX>(h, op, k, h.ro.p, true, 0.0, inner_maxit, false, h.st.p + 1);
     hipLaunchKernelGGL((k_add_correction<TX>), dim3(blocks_for((int64_t)h.N(), kVecThreads)), dim3(kVecThreads), 0,
                        h.stream, (int64_t)h.N(), h.xo.p, (const TX *)k.x.p);
     check_launch();
+    hipLaunchKernelGGL(k_refine_step, dim3(1), dim3(1), 0, h.stream, h.st.p + 1, (const SolverState *)h.st.p);
+    check_launch();
+    return o.aborted;
 }
 
 template <int BS>
@@ -678,77 +715,57 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     const auto t0 = std::chrono::steady_clock::now();
     gather_vec(h, d_b, h.bp.p);
     gather_vec(h, d_x, h.xo.p);
-    OpSet<double> op64 = op_fwd64(h);
-    bda_result r;
-    std::memset(&r, 0, sizeof(r));
-    double nrm0 = 0.0, thr = 0.0;
-    int slot = 1;
+    SolverState os = new_state(h, 1, tol, 0, h.abs_tol != 0);
+    os.max_outer = max_outer;
+    os.inner_tol = inner_tol;
+    os.eta = h.refine_eta;
+    os.adaptive = h.refine_adaptive;
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, h.stream, h.st.p + 1, os);
+    check_launch();
+    bool aborted = false;
     for (int outer = 0;; ++outer) {
-        // FP64 true residual ro = b - A x
-        init_state(h, 1, 0.0, 0, false);
+        // FP64 true residual ro = b - A x and the outer step's decisions
         prof_begin(h, PK_SPMV);
         launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
-                                                           nullptr, red(h, 1, FIN_STORE), true);
+                                                           nullptr, red(h, 1, FIN_REFINE), true);
         prof_end(h);
-        SolverState s1;
-        HIP_CHECK(hipMemcpyAsync(h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost, h.stream));
-        HIP_CHECK(hipStreamSynchronize(h.stream));
-        s1 = *h.h_st;
-        const double nrm = std::sqrt(s1.tot[0]);
-        if (outer == 0) {
-            nrm0 = nrm;
-            r.norms[0] = nrm;
-            const double precision = h.abs_tol ? tol : kAlreadySolved;
-            if (!(nrm > precision)) {
-                r.no_results = 1;
-                if (std::isnan(nrm)) r.nan_detected = 1;
-                break;
-            }
-            thr = h.abs_tol ? tol : tol * nrm;
-        } else {
-            r.norms[slot] = nrm;
-            r.last_norm_idx = slot;
-            slot = slot == 3 ? 1 : slot + 1;
-        }
-        r.final_norm = nrm;
-        if (std::isnan(nrm) || std::isinf(nrm)) {
-            r.nan_detected = 1;
+        if (outer == max_outer) break;
+        HIP_CHECK(hipEventRecord(h.poll_ev[0], h.stream));
+        HIP_CHECK(hipEventSynchronize(h.poll_ev[0]));
+        if (run_stopped(h, 1, os.run_id)) break;
+        const bool ab = vec32 ? mixed_inner<BS, float>(h, h.k32, inner_maxit)
+                              : mixed_inner<BS, double>(h, h.k64, inner_maxit);
+        if (ab) {
+            aborted = true;
             break;
         }
-        if (outer > 0 && !(nrm > thr)) {
-            r.converged = 1;
+        if (h.timeout_s > 0.0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > h.timeout_s) {
+            aborted = true;
             break;
         }
-        if (outer == max_outer) {
-            r.max_iter_reached = 1;
-            break;
-        }
-        // inner tolerance: inner_tol, or -- adaptive -- only the reduction the
-        // outer target still needs (eta * thr / ||r_k||), whichever is looser
-        double itol = inner_tol;
-        if (h.refine_adaptive) itol = std::min(0.5, std::max(inner_tol, h.refine_eta * thr / nrm));
-        RunOut o;
-        if (vec32) mixed_inner<BS, float>(h, h.k32, itol, inner_maxit, o);
-        else mixed_inner<BS, double>(h, h.k64, itol, inner_maxit, o);
-        r.outer_iterations++;
-        r.inner_half_iterations += o.st.h;
-        r.iterations += o.st.h / 2.0 + 0.5;
-        if (o.st.breakdown || o.st.nan_flag) {
-            r.breakdown = o.st.breakdown;
-            r.nan_detected = o.st.nan_flag;
-            // the correction was still applied; report the FP64 residual after it
-        }
-        if (o.aborted) {
-            r.aborted = 1;
-            break;
-        }
-        if (o.st.no_results) break;  // inner rhs already zero
     }
-    (void)op64;
     scatter_vec(h, h.xo.p, d_x);
     HIP_CHECK(hipStreamSynchronize(h.stream));
-    r.rel_residual = nrm0 > 0 ? r.final_norm / nrm0 : 0.0;
-    r.half_iterations = r.inner_half_iterations;
+    prof_collect(h);  // inner runs are not filtered: their no-op launches count too
+    HIP_CHECK(hipMemcpy(h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost));
+    const SolverState s = *h.h_st;
+    bda_result r;
+    std::memset(&r, 0, sizeof(r));
+    for (int i = 0; i < 4; ++i) r.norms[i] = s.norms[i];
+    r.last_norm_idx = s.last_norm_idx;
+    r.final_norm = s.last_norm;
+    r.converged = s.converged;
+    r.no_results = s.no_results;
+    r.max_iter_reached = s.max_iter;
+    r.breakdown = s.breakdown;
+    r.nan_detected = s.nan_flag;
+    r.aborted = aborted ? 1 : 0;
+    r.outer_iterations = s.outer;
+    r.inner_half_iterations = s.inner_h;
+    r.half_iterations = s.inner_h;
+    r.iterations = s.iters;
+    r.rel_residual = s.nrm0 > 0 ? s.last_norm / s.nrm0 : 0.0;
     r.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (res) *res = r;
     h.last_solve_valid = true;
@@ -881,7 +898,8 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         HIP_CHECK(hipStreamCreate(&h->stream));
         h->st.ensure(2);
         HIP_CHECK(hipHostMalloc((void **)&h->h_st, sizeof(SolverState), hipHostMallocDefault));
-        HIP_CHECK(hipHostMalloc((void **)&h->h_done, sizeof(int32_t), hipHostMallocMapped));
+        HIP_CHECK(hipHostMalloc((void **)&h->h_done, 2 * sizeof(int32_t), hipHostMallocMapped));
+        h->h_done[0] = h->h_done[1] = 0;
         HIP_CHECK(hipHostGetDevicePointer((void **)&h->d_done, h->h_done, 0));
         for (auto &e : h->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         h->d_orig.ensure((size_t)nnzb_max * bs * bs);

@@ -199,3 +199,32 @@ def test_update_values_reuses_handle():
     assert np.linalg.norm(x2 - xo) / np.linalg.norm(xo) < 1e-8
     assert np.linalg.norm(x2 * 1.5 - x1) / np.linalg.norm(x1) < 1e-8
     g.close()
+
+
+def test_mixed_edge_cases():
+    """Stop rules of the device-side refinement loop against the oracle's."""
+    s, g, o = _pair(scaled(GRID_32x32x16, 8, 7, 5))
+    # zero right-hand side: nothing to do
+    x, r = g.solve_mixed(np.zeros(s.N), tol=1e-6)
+    xo, ro = o.solve_mixed(np.zeros(s.N), tol=1e-6)
+    assert r.no_results and ro["no_results"] and not r.converged
+    assert r.outer_iterations == 0 and np.all(x == 0)
+    # no outer step allowed: the residual of x0 only
+    x, r = g.solve_mixed(s.b, tol=1e-6, max_outer=0)
+    xo, ro = o.solve_mixed(s.b, tol=1e-6, max_outer=0)
+    assert r.max_iter_reached and ro["max_iter_reached"] and r.outer_iterations == 0
+    assert np.all(x == 0) and np.isclose(r.norms[0], ro["norms"][0], rtol=1e-12)
+    # one outer step towards an unreachable target
+    x, r = g.solve_mixed(s.b, tol=1e-14, max_outer=1)
+    xo, ro = o.solve_mixed(s.b, tol=1e-14, max_outer=1)
+    assert r.max_iter_reached and ro["max_iter_reached"] and not r.converged
+    assert r.outer_iterations == ro["outer_iterations"] == 1 and r.last_norm_idx == ro["last_norm_idx"] == 1
+    assert np.isclose(r.final_norm, ro["final_norm"], rtol=1e-3)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-3
+    # runs of both kinds back to back on one handle keep their own results
+    for _ in range(2):
+        x64, r64 = g.solve(s.b, tol=1e-8, maxit=300)
+        xm, rm = g.solve_mixed(s.b, tol=1e-8)
+        assert r64.converged and rm.converged and rm.outer_iterations >= 1
+        assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-6
+    g.close()
