@@ -60,6 +60,15 @@ class BdaGridSpec(C.Structure):
     ]
 
 
+class BdaWireSizes(C.Structure):
+    _fields_ = [
+        ("vector_sizes", i32 * 18),
+        ("nnz_val_sizes", i32 * 3),
+        ("partitions", i32 * 3),
+        ("device_fits", i32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/bda_bicgstab.h declares
 SIGNATURES = {
     "bda_grid_create": (i32, [C.POINTER(BdaGridSpec), C.POINTER(vp)]),
@@ -99,6 +108,15 @@ SIGNATURES = {
     "bda_get_kernel_stats": (i32, [vp, i32, C.POINTER(i64), P_f64]),
     "bda_time_spmv": (i32, [vp, i32, i32, P_f64]),
     "bda_algorithmic_bytes": (i32, [vp, i32, i32, P_f64]),
+    "bda_wire_export_sizes": (i32, [vp, C.POINTER(BdaWireSizes)]),
+    "bda_wire_export": (i32, [vp, vp, vp, vp]),
+    "bda_wire_build_sizes": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, C.POINTER(BdaWireSizes)]),
+    "bda_wire_build": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "bda_wire_layout": (i32, [vp, vp, C.c_uint32, vp, vp]),
+    "bda_wire_pack": (i32, [vp, vp, C.c_uint32, vp, vp]),
+    "bda_wire_unpack": (i32, [vp, vp, vp, vp, vp]),
+    "bda_wire_import_sizes": (i32, [i32, vp, vp, P_i32, C.POINTER(i64), P_i32]),
+    "bda_wire_import": (i32, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
 }
 
 

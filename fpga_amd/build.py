@@ -28,7 +28,7 @@ LIB = PKG / "libfpga_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-SOURCES = ["grid_gen.cpp", "analysis.cpp", "host_capi.cpp", "solver.hip"]
+SOURCES = ["grid_gen.cpp", "analysis.cpp", "host_capi.cpp", "wire_format.cpp", "solver.hip"]
 TORCH_SRC = "torch_adjoint.cpp"
 TORCH_EXT = "_torch_bda"
 

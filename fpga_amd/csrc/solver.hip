@@ -1156,6 +1156,59 @@ int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *
     });
 }
 
+// Reference wire format of the handle's analysed and factored system
+// (host arrays; the device copies of A, L, U and D^-1 are read back).
+namespace {
+struct WireInputs {
+    std::vector<double> a, l, u, dinv;
+};
+void wire_inputs(H &h, WireInputs &w) {
+    require_factored(h);
+    const Analysis &an = h.an;
+    const size_t bb = (size_t)h.bs * h.bs;
+    w.a.resize(an.A.nnzb() * bb);
+    w.l.resize(std::max<size_t>(an.L.nnzb() * bb, 1));
+    w.u.resize(std::max<size_t>(an.U.nnzb() * bb, 1));
+    w.dinv.resize((size_t)h.n * bb);
+    HIP_CHECK(hipMemcpy(w.a.data(), h.A64.p, w.a.size() * 8, hipMemcpyDeviceToHost));
+    if (an.L.nnzb()) HIP_CHECK(hipMemcpy(w.l.data(), h.L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost));
+    if (an.U.nnzb()) HIP_CHECK(hipMemcpy(w.u.data(), h.U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(w.dinv.data(), h.Dinv.p, w.dinv.size() * 8, hipMemcpyDeviceToHost));
+}
+std::vector<double> to_colour_order(const H &h, const double *v) {
+    std::vector<double> out((size_t)h.n * h.bs, 0.0);
+    if (!v) return out;
+    for (int k = 0; k < h.n; ++k)
+        for (int a = 0; a < h.bs; ++a) out[(size_t)k * h.bs + a] = v[(size_t)h.an.perm[k] * h.bs + a];
+    return out;
+}
+}  // namespace
+
+int bda_wire_export_sizes(bda_handle h, bda_wire_sizes *ws) {
+    return guarded(h, [&] {
+        require_factored(*h);
+        const Analysis &an = h->an;
+        if (bda_wire_build_sizes(h->bs, h->n, an.colour_ptr.data(), an.ncolours, an.A.rowptr.data(),
+                                 an.A.colidx.data(), an.L.rowptr.data(), an.L.colidx.data(), an.U.rowptr.data(),
+                                 an.U.colidx.data(), ws) != 0)
+            throw std::runtime_error("wire format sizes failed");
+    });
+}
+
+int bda_wire_export(bda_handle h, const double *b, const double *x0, void *vector_pointers[21]) {
+    return guarded(h, [&] {
+        WireInputs w;
+        wire_inputs(*h, w);
+        const Analysis &an = h->an;
+        const std::vector<double> bp = to_colour_order(*h, b), xp = to_colour_order(*h, x0);
+        if (bda_wire_build(h->bs, h->n, an.colour_ptr.data(), an.ncolours, an.A.rowptr.data(), an.A.colidx.data(),
+                           w.a.data(), an.L.rowptr.data(), an.L.colidx.data(), w.l.data(), an.U.rowptr.data(),
+                           an.U.colidx.data(), w.u.data(), w.dinv.data(), bp.data(), xp.data(),
+                           vector_pointers) != 0)
+            throw std::runtime_error("wire format export failed");
+    });
+}
+
 int bda_set_profiling(bda_handle h, int32_t on) {
     return guarded(h, [&] {
         h->profiling = on != 0;

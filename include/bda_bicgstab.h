@@ -219,6 +219,73 @@ int bda_factor_sizes(bda_handle h, int32_t *nnzb_lower, int32_t *nnzb_upper);
 int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *l_values,
                     int32_t *u_rowptr, int32_t *u_colidx, double *u_values, double *dinv);
 
+/* ---- the reference's per-colour device format (host only) ---------------
+ * The arrays the reference's host library copies into the FPGA's data
+ * buffers: vector_pointers[0..20] and vectorSizes[0..17] as
+ * fpga_copy_host_datamem reads them (fpga_functions_bicgstab.cpp:506-629),
+ * for A (slots 0-4), L (6-10), U (12-16), D^-1 (18), b (19) and x0 (20):
+ *   [6k+0] int32[8 + 4*parts]  8 header words, then {row, arow, col, val}
+ *   [6k+1] uint32[sizes[6k+3]] gathered-entry (P) indices, 16 per partition
+ *   [6k+2] double*[1]          -> double[nnz_val_sizes[k]] stream values
+ *   [6k+3] uint16[sizes[6k+1]] slot of each stream entry in the P list
+ *   [6k+4] uint8[sizes[6k+4]]  row advance of each entry, 64 per partition
+ *   [18]   double[sizes[5]]    D^-1, 4 per scalar row of U's partitions
+ *   [19], [20] double[N]       b and x0 in colour order
+ * A "colour" of the format is a partition of a colour within the device
+ * limits (constants.vhd:80-98: 2048 rows, 8192 gathered entries, 32768
+ * entries).  device_fits: the system also fits the U280 design as a whole
+ * (N <= 163840, <= 256 partitions and <= 2^21 entries per matrix). */
+typedef struct bda_wire_sizes {
+    int32_t vector_sizes[18];
+    int32_t nnz_val_sizes[3];
+    int32_t partitions[3];
+    int32_t device_fits;
+} bda_wire_sizes;
+
+/* Replaces the reference's consumer-side inputs (fpga_functions_bicgstab.
+ * cpp:606-629): builds the arrays from the handle's analysed and factored
+ * system; b and x0 in the caller's ordering (either may be NULL: zeros). */
+int bda_wire_export_sizes(bda_handle h, bda_wire_sizes *ws);
+int bda_wire_export(bda_handle h, const double *b, const double *x0, void *vector_pointers[21]);
+
+/* The same from colour-ordered host arrays (no handle, no GPU): A, L, U as
+ * block CSR with sorted columns, dinv[n*bs*bs], b_perm and x0_perm in colour
+ * order (may be NULL). */
+int bda_wire_build_sizes(int32_t bs, int32_t n, const int32_t *colour_ptr, int32_t ncolours,
+                         const int32_t *a_rowptr, const int32_t *a_colidx, const int32_t *l_rowptr,
+                         const int32_t *l_colidx, const int32_t *u_rowptr, const int32_t *u_colidx,
+                         bda_wire_sizes *ws);
+int bda_wire_build(int32_t bs, int32_t n, const int32_t *colour_ptr, int32_t ncolours, const int32_t *a_rowptr,
+                   const int32_t *a_colidx, const double *a_values, const int32_t *l_rowptr,
+                   const int32_t *l_colidx, const double *l_values, const int32_t *u_rowptr,
+                   const int32_t *u_colidx, const double *u_values, const double *dinv, const double *b_perm,
+                   const double *x0_perm, void *vector_pointers[21]);
+
+/* The five data buffers and the 5-line setup block of the FPGA kernel:
+ * fpga_setup_host_datamem's layout (fpga_functions_bicgstab.cpp:84-496) and
+ * fpga_copy_host_datamem's copy (:506-629).  bda_wire_layout gives the
+ * buffer sizes in bytes and the 40 setup words; bda_wire_pack fills
+ * caller-allocated buffers (the dump_input_data_<b>_seq_<s>.bin files of
+ * :649-665 are these buffers); bda_wire_unpack points vector_pointers into
+ * packed buffers (value_arrays holds the three double* the double** slots
+ * point to). */
+int bda_wire_layout(const int32_t vector_sizes[18], const int32_t nnz_val_sizes[3], uint32_t config_bits,
+                    uint32_t total_size[5], uint64_t setup[40]);
+int bda_wire_pack(const int32_t vector_sizes[18], const int32_t nnz_val_sizes[3], uint32_t config_bits,
+                  void *const vector_pointers[21], uint8_t *buffers[5]);
+int bda_wire_unpack(uint8_t *const buffers[5], int32_t vector_sizes[18], int32_t nnz_val_sizes[3],
+                    void *vector_pointers[21], double *value_arrays[3]);
+
+/* Import: the arrays back into colour-ordered block CSR.  A's pattern is
+ * the union of the blocks its stream touches; lu_values (on A's pattern)
+ * receives L on the strictly lower and U on the strictly upper blocks;
+ * part_rows[parts+1] the first scalar row of each of A's partitions. */
+int bda_wire_import_sizes(int32_t bs, const int32_t vector_sizes[18], void *const vector_pointers[21],
+                          int32_t *n, int64_t *nnzb, int32_t *num_parts);
+int bda_wire_import(int32_t bs, const int32_t vector_sizes[18], void *const vector_pointers[21], int32_t *rowptr,
+                    int32_t *colidx, double *values, double *lu_values, double *dinv, int32_t *part_rows,
+                    double *b, double *x0);
+
 /* ---- measurement ------------------------------------------------------ */
 /* Kernel classes timed with HIP events on the handle's stream while
  * profiling is on: 0 = SpMV, 1 = lower sweep, 2 = upper sweep, 3 = vector
