@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bda_host.hpp"  // kTileRows
 
 namespace bda {
@@ -384,6 +386,19 @@ template <> struct TileCap<4, double> { static constexpr int value = 128; };  //
 template <> struct TileCap<3, float> { static constexpr int value = 336; };   // FP32 values: 16 steps
 template <> struct TileCap<4, float> { static constexpr int value = 256; };
 
+// Which tiling a kernel instance uses: the wide FP32 tiling for FP32-value
+// kernels in the modes whose bit is set in BDA_WIDE_MODES, the FP64 tiling
+// otherwise.
+#ifndef BDA_WIDE_MODES
+#define BDA_WIDE_MODES 1
+#endif
+template <typename TV, int MODE> constexpr bool wide_tiles() {
+    return std::is_same<TV, float>::value && ((BDA_WIDE_MODES >> MODE) & 1);
+}
+template <int BS, typename TV, int MODE> constexpr int stream_cap() {
+    return wide_tiles<TV, MODE>() ? TileCap<BS, float>::value : TileCap<BS, double>::value;
+}
+
 // Matrix values and column indices are read once per launch and use
 // non-temporal loads.  (The x entries are still fetched into L2 three to four
 // times per launch: the measured fetch volume is the same with default,
@@ -418,7 +433,7 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
-    constexpr int CAP = TileCap<BS, TV>::value;
+    constexpr int CAP = stream_cap<BS, TV, MODE>();
     constexpr int BPW = 64 / BB;           // blocks per wave step (7 or 4)
     constexpr int VPW = BPW * BB;          // lanes carrying a value (63 or 64)
     constexpr int S = CAP / (NW * BPW);    // value slots per lane

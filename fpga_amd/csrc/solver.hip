@@ -268,7 +268,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
                    TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
-    constexpr bool f32 = std::is_same<TV, float>::value;
+    constexpr bool f32 = wide_tiles<TV, MODE>();
     const std::vector<int32_t> &ct = f32 ? m.colour_tile32 : m.colour_tile;
     a.tile = f32 ? m.tile32.p : m.tile.p;
     if (colour < 0) {
@@ -304,7 +304,7 @@ template <int BS, typename TV, typename TX>
 void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, RedArgs ra) {
     StreamArgs<TV, TX> a;
     std::memset(&a, 0, sizeof(a));
-    constexpr bool f32 = std::is_same<TV, float>::value;
+    constexpr bool f32 = wide_tiles<TV, SM_SPMV>();
     a.tile = f32 ? h.A.tile32.p : h.A.tile.p;
     a.tile0 = 0;
     a.ntiles = f32 ? h.A.ntiles32 : h.A.ntiles;
