@@ -227,7 +227,7 @@ def main():
                                                     "vector_updates_per_half_iter": ms_vec / max(n_vec, 1)}},
         "roofline": {"bound": "hbm", "achieved": spmv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": spmv_gbs / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stream<3, SM_SPMV, SR_W0, double, double>",
+                     "kernel": f"k_stream<{s.bs}, SM_SPMV, SR_W0, double, double>",
                      "algorithmic_bytes_per_launch": spmv_bytes},
         "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac},
         "adjoint": adj,
