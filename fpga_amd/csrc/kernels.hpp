@@ -450,10 +450,13 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     __shared__ int32_t srow[NT];
     __shared__ double tv[MODE == SM_SUB_DINV ? NT : 1];
 
-    if (a.ra.st != nullptr && a.ra.st->done) return;
-
+    // the stop flag and the tile descriptor are loaded together (one scalar
+    // round trip instead of two before the first vector load): the exit test
+    // also reads the descriptor (a negative entry never occurs), so the
+    // compiler cannot sink its load below the branch
     const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
     const int2 d0 = a.tile[tile], d1 = a.tile[tile + 1];
+    if ((a.ra.st->done | ((d0.x | d0.y | d1.x | d1.y) < 0 ? 1 : 0)) != 0) return;
     const int r0 = d0.x, r1 = d1.x, b0 = d0.y;
     const int nblk = d1.y - b0;
     const int nrow = r1 - r0;

@@ -234,6 +234,11 @@ void prof_collect(H &h, int max_tag = INT_MAX) {
 
 void check_launch() { HIP_CHECK(hipGetLastError()); }
 
+// State slots: 0 = the running BiCGSTAB, 1 = the outer refinement loop (or a
+// standalone reduction), kIdleState = a state that is never stopped, for
+// launches outside a solve (the streaming kernel always reads a state).
+constexpr int kIdleState = 2;
+
 RedArgs red(H &h, int state_idx, int fin) {
     RedArgs ra;
     ra.partials = h.partials.p;
@@ -292,7 +297,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     a.dinv = dinv;
     a.w0 = w0;
     a.ra = ra;
-    if (!use_state) a.ra.st = nullptr;
+    if (!use_state) a.ra.st = h.st.p + kIdleState;
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
     if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
@@ -896,7 +901,7 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         // the legacy default stream (torch's default), so device inputs
         // produced there are complete before the handle reads them
         HIP_CHECK(hipStreamCreate(&h->stream));
-        h->st.ensure(2);
+        h->st.ensure(kIdleState + 1);  // DBuf zero-fills: the idle slot never has done set
         HIP_CHECK(hipHostMalloc((void **)&h->h_st, sizeof(SolverState), hipHostMallocDefault));
         HIP_CHECK(hipHostMalloc((void **)&h->h_done, 2 * sizeof(int32_t), hipHostMallocMapped));
         h->h_done[0] = h->h_done[1] = 0;
