@@ -45,6 +45,7 @@ struct SolverState {
     int32_t last_half;      // 1: s is the latest residual, 2: r is
     int32_t run_id;         // written to host_done when the run stops
     int32_t *host_done;     // mapped host memory the host polls
+    uint64_t *host_prog;    // mapped: run_id << 32 | progress (half iterations, or outer steps)
     // outer refinement of the precision form (used on the outer state)
     double nrm0;            // FP64 residual norm of the start vector
     double itol;            // inner tolerance for the next inner run
@@ -93,6 +94,15 @@ __device__ inline bool finite(double v) { return !__builtin_isnan(v) && !__built
 __device__ inline void signal_done(SolverState *st) {
     st->done = 1;
     if (st->host_done) __hip_atomic_store(st->host_done, st->run_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Progress for the host's throttling (it keeps a bounded number of
+// iterations queued ahead of the device): a mapped word, so no event has to
+// be recorded in the stream.
+__device__ inline void signal_progress(SolverState *st, int32_t count) {
+    if (st->host_prog)
+        __hip_atomic_store(st->host_prog, ((uint64_t)(uint32_t)st->run_id << 32) | (uint32_t)count, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Scalar epilogues; run by one thread after a grid-wide reduction.
@@ -158,7 +168,10 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
             st->rho_prev = st->rho;
             st->rho = t1;
             if (t1 == 0.0 || st->omega == 0.0) { st->breakdown = 1; signal_done(st); }
-            else st->beta = (st->rho / st->rho_prev) * (st->alpha / st->omega);
+            else {
+                st->beta = (st->rho / st->rho_prev) * (st->alpha / st->omega);
+                signal_progress(st, st->h);
+            }
         }
         break;
     }
@@ -189,6 +202,7 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         } else {
             st->itol = st->inner_tol;
         }
+        signal_progress(st, st->outer + 1);
         break;
     }
     default:

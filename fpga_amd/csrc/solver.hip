@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -163,6 +164,8 @@ struct bda_handle_s {
     bda::SolverState *h_st = nullptr;  // pinned
     int32_t *h_done = nullptr;          // mapped host flags, one per state: the id of the run that stopped
     int32_t *d_done = nullptr;          // their device address
+    uint64_t *h_prog = nullptr;         // mapped progress words, one per state (run id << 32 | count)
+    uint64_t *d_prog = nullptr;
     int32_t run_seq = 0;                // last run id handed out
     hipEvent_t poll_ev[4] = {};
 
@@ -388,11 +391,29 @@ SolverState new_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
     h.run_seq = h.run_seq == INT32_MAX ? 1 : h.run_seq + 1;
     s.run_id = h.run_seq;
     s.host_done = h.d_done + idx;
+    s.host_prog = h.d_prog + idx;
     return s;
 }
 
 bool run_stopped(const H &h, int idx, int32_t run_id) {
     return __atomic_load_n(h.h_done + idx, __ATOMIC_ACQUIRE) == run_id;
+}
+
+// Waits until run run_id on state slot idx has reported progress >= count,
+// or has stopped; false if the handle's timeout (measured from t0) expired.
+bool wait_progress(const H &h, int idx, int32_t run_id, int32_t count,
+                   std::chrono::steady_clock::time_point t0) {
+    for (uint32_t spin = 0;; ++spin) {
+        const uint64_t v = __atomic_load_n(h.h_prog + idx, __ATOMIC_ACQUIRE);
+        if ((int32_t)(v >> 32) == run_id && (int32_t)(uint32_t)v >= count) return true;
+        if (run_stopped(h, idx, run_id)) return true;
+        if ((spin & 1023) == 1023) {
+            if (h.timeout_s > 0.0 &&
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > h.timeout_s)
+                return false;
+            std::this_thread::yield();
+        }
+    }
 }
 
 int32_t init_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
@@ -474,8 +495,13 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                            k.v.p, (const SolverState *)(h.st.p));
         check_launch();
         prof_end(h);
-        HIP_CHECK(hipEventRecord(h.poll_ev[it & 3], h.stream));
-        if (it > ahead) HIP_CHECK(hipEventSynchronize(h.poll_ev[(it - ahead) & 3]));
+        // keep at most `ahead` iterations queued beyond the one running:
+        // wait (on the mapped progress word, no stream event) until the
+        // device has finished iteration it - ahead
+        if (it > ahead && !wait_progress(h, 0, run_id, 2 * (it - ahead), t0)) {
+            out.aborted = true;
+            stop = true;
+        }
         if (run_stopped(h, 0, run_id)) stop = true;
         if (h.timeout_s > 0.0) {
             const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -735,8 +761,10 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
                                                            nullptr, red(h, 1, FIN_REFINE), true);
         prof_end(h);
         if (outer == max_outer) break;
-        HIP_CHECK(hipEventRecord(h.poll_ev[0], h.stream));
-        HIP_CHECK(hipEventSynchronize(h.poll_ev[0]));
+        if (!wait_progress(h, 1, os.run_id, outer + 1, t0)) {
+            aborted = true;
+            break;
+        }
         if (run_stopped(h, 1, os.run_id)) break;
         const bool ab = vec32 ? mixed_inner<BS, float>(h, h.k32, inner_maxit)
                               : mixed_inner<BS, double>(h, h.k64, inner_maxit);
@@ -906,6 +934,9 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         HIP_CHECK(hipHostMalloc((void **)&h->h_done, 2 * sizeof(int32_t), hipHostMallocMapped));
         h->h_done[0] = h->h_done[1] = 0;
         HIP_CHECK(hipHostGetDevicePointer((void **)&h->d_done, h->h_done, 0));
+        HIP_CHECK(hipHostMalloc((void **)&h->h_prog, 2 * sizeof(uint64_t), hipHostMallocMapped));
+        h->h_prog[0] = h->h_prog[1] = 0;
+        HIP_CHECK(hipHostGetDevicePointer((void **)&h->d_prog, h->h_prog, 0));
         for (auto &e : h->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         h->d_orig.ensure((size_t)nnzb_max * bs * bs);
     } catch (const std::exception &e) {
@@ -933,6 +964,7 @@ int bda_destroy(bda_handle h) {
     h->st.release();
     if (h->h_st) (void)hipHostFree(h->h_st);
     if (h->h_done) (void)hipHostFree(h->h_done);
+    if (h->h_prog) (void)hipHostFree(h->h_prog);
     for (auto &e : h->poll_ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &pr : h->ev_pool) {
