@@ -108,6 +108,8 @@ SIGNATURES = {
     "bda_get_kernel_stats": (i32, [vp, i32, C.POINTER(i64), P_f64]),
     "bda_time_spmv": (i32, [vp, i32, i32, P_f64]),
     "bda_algorithmic_bytes": (i32, [vp, i32, i32, P_f64]),
+    "bda_set_history": (i32, [vp, i32]),
+    "bda_get_history": (i32, [vp, vp, i32, P_i32]),
     "bda_wire_export_sizes": (i32, [vp, C.POINTER(BdaWireSizes)]),
     "bda_wire_export": (i32, [vp, vp, vp, vp]),
     "bda_wire_build_sizes": (i32, [i32, i32, vp, i32, vp, vp, vp, vp, vp, vp, C.POINTER(BdaWireSizes)]),

@@ -46,6 +46,8 @@ struct SolverState {
     int32_t run_id;         // written to host_done when the run stops
     int32_t *host_done;     // mapped host memory the host polls
     uint64_t *host_prog;    // mapped: run_id << 32 | progress (half iterations, or outer steps)
+    double *hist;           // optional norm history: ||r0||, then every half-iteration's norm
+    int32_t hist_cap, nhist;
     // outer refinement of the precision form (used on the outer state)
     double nrm0;            // FP64 residual norm of the start vector
     double itol;            // inner tolerance for the next inner run
@@ -79,7 +81,13 @@ struct RedArgs {
     int32_t fin;
 };
 
+__device__ inline void record_norm(SolverState *st, double nrm) {
+    if (st->hist != nullptr && st->nhist < st->hist_cap) st->hist[st->nhist] = nrm;
+    st->nhist += 1;
+}
+
 __device__ inline void push_norm(SolverState *st, double nrm) {
+    record_norm(st, nrm);
     const int slot = st->norm_slot;
     st->norms[slot] = nrm;
     st->last_norm_idx = slot;
@@ -114,6 +122,8 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         break;
     case FIN_INIT: {   // solver.vhd:867-936 with dot_res = rho_new
         const double nrm = sqrt(t0);
+        st->nhist = 0;
+        record_norm(st, nrm);
         st->rho = t0;
         st->norms[0] = nrm;
         st->norm_slot = 1;

@@ -166,6 +166,8 @@ struct bda_handle_s {
     int32_t *d_done = nullptr;          // their device address
     uint64_t *h_prog = nullptr;         // mapped progress words, one per state (run id << 32 | count)
     uint64_t *d_prog = nullptr;
+    bda::DBuf<double> hist;             // norm history of the runs on state slot 0 (optional)
+    int32_t hist_cap = 0;
     int32_t run_seq = 0;                // last run id handed out
     hipEvent_t poll_ev[4] = {};
 
@@ -392,6 +394,10 @@ SolverState new_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
     s.run_id = h.run_seq;
     s.host_done = h.d_done + idx;
     s.host_prog = h.d_prog + idx;
+    if (idx == 0 && h.hist_cap > 0) {
+        s.hist = h.hist.p;
+        s.hist_cap = h.hist_cap;
+    }
     return s;
 }
 
@@ -1243,6 +1249,31 @@ int bda_wire_export(bda_handle h, const double *b, const double *x0, void *vecto
                            an.U.colidx.data(), w.u.data(), w.dinv.data(), bp.data(), xp.data(),
                            vector_pointers) != 0)
             throw std::runtime_error("wire format export failed");
+    });
+}
+
+int bda_set_history(bda_handle h, int32_t capacity) {
+    return guarded(h, [&] {
+        if (capacity < 0) throw std::runtime_error("negative history capacity");
+        h->hist.release();
+        h->hist_cap = 0;
+        if (capacity > 0) {
+            h->hist.ensure((size_t)capacity);
+            h->hist_cap = capacity;
+        }
+    });
+}
+
+int bda_get_history(bda_handle h, double *norms, int32_t capacity, int32_t *count) {
+    return guarded(h, [&] {
+        if (!h->last_solve_valid) throw std::runtime_error("no solve has run");
+        HIP_CHECK(hipStreamSynchronize(h->stream));
+        SolverState s;
+        HIP_CHECK(hipMemcpy(&s, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+        const int32_t avail = std::min(s.nhist, h->hist_cap);
+        const int32_t n = std::max(0, std::min(avail, capacity));
+        if (n > 0 && norms) HIP_CHECK(hipMemcpy(norms, h->hist.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (count) *count = s.nhist;
     });
 }
 

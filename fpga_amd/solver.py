@@ -332,6 +332,17 @@ class BicgstabSolver:
         self._check(lib.bda_get_kernel_stats(self._h, kind, C.byref(n), C.byref(ms)), "bda_get_kernel_stats")
         return n.value, ms.value
 
+    def set_history(self, capacity: int):
+        """Record ||r0|| and every half-iteration's residual norm (up to capacity values)."""
+        self._check(lib.bda_set_history(self._h, capacity), "bda_set_history")
+
+    def history(self, capacity: int = 1 << 16) -> np.ndarray:
+        out = np.empty(max(capacity, 1))
+        n = C.c_int32()
+        self._check(lib.bda_get_history(self._h, out.ctypes.data_as(C.c_void_p), capacity, C.byref(n)),
+                    "bda_get_history")
+        return out[:min(n.value, capacity)].copy()
+
     def time_spmv(self, reps: int = 20, fp32: bool = False) -> float:
         ms = C.c_double()
         self._check(lib.bda_time_spmv(self._h, reps, int(fp32), C.byref(ms)), "bda_time_spmv")

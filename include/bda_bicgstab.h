@@ -219,6 +219,17 @@ int bda_factor_sizes(bda_handle h, int32_t *nnzb_lower, int32_t *nnzb_upper);
 int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *l_values,
                     int32_t *u_rowptr, int32_t *u_colidx, double *u_values, double *dinv);
 
+/* ---- norm history ------------------------------------------------------
+ * Replaces the reference's periodic debug lines, which carry the last four
+ * norms every debug_rate cycles (solver.vhd:1074-1148): with a capacity set,
+ * every run on the handle's main state records ||r0|| and then the residual
+ * norm of every half iteration, in order, on the device.  bda_get_history
+ * copies up to `capacity` of them from the last run (for the precision form:
+ * the last inner run) and reports how many the run produced (h + 2 for a run
+ * that stopped after h half iterations). */
+int bda_set_history(bda_handle h, int32_t capacity);
+int bda_get_history(bda_handle h, double *norms, int32_t capacity, int32_t *count);
+
 /* ---- the reference's per-colour device format (host only) ---------------
  * The arrays the reference's host library copies into the FPGA's data
  * buffers: vector_pointers[0..20] and vectorSizes[0..17] as

@@ -116,6 +116,7 @@ def lib():
             "orc_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, _i32, _f64, C.POINTER(OrcResult)],
             "orc_adjoint_solve": [_vp, _vp, _vp, _f64, _i32, C.POINTER(OrcResult)],
             "orc_grad": [_vp, _vp, _vp, _vp],
+            "orc_get_history": [_vp, _vp, _i32, C.POINTER(_i32)],
         }.items():
             f = getattr(L, name)
             f.restype = _i32
@@ -230,6 +231,12 @@ class OracleSystem:
                                  int(adaptive), eta, C.byref(r)) != 0:
             raise RuntimeError("orc_solve_mixed failed")
         return x, r.as_dict()
+
+    def history(self, capacity=1 << 16):
+        out = np.empty(capacity)
+        n = _i32()
+        lib().orc_get_history(self._h, _p(out), capacity, C.byref(n))
+        return out[:min(n.value, capacity)].copy()
 
     def adjoint_solve(self, w, lam0=None, tol=1e-10, maxit=1000):
         w = np.ascontiguousarray(w, np.float64)

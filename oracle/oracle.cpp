@@ -215,7 +215,9 @@ struct RunState {
     double norms[4] = {0, 0, 0, 0};
     double last = 0.0;
     bool converged = false, no_results = false, max_iter = false, breakdown = false, nan = false;
+    std::vector<double> hist;  // ||r0||, then every half-iteration's norm, in order
     void push(double v) {
+        hist.push_back(v);
         norms[norm_slot] = v;
         last_idx = norm_slot;
         norm_slot = norm_slot == 3 ? 1 : norm_slot + 1;
@@ -257,6 +259,7 @@ RunState bicgstab(const OpSet<TV> &op, int64_t N, const double *rhs, TX *x, bool
     const double n0 = std::sqrt(rho);
     st.norms[0] = n0;
     st.last = n0;
+    st.hist.push_back(n0);
     if (!(n0 > (abs_mode ? tol : kAlreadySolved))) {
         st.no_results = true;
         st.nan = !finite(n0);
@@ -354,6 +357,7 @@ struct orc_system {
     bool adj_ready = false;
     BMat<float> A32, L32, U32;
     bool mixed_ready = false;
+    std::vector<double> last_hist;     // norm history of the last orc_solve / orc_adjoint_solve
 };
 
 extern "C" {
@@ -624,6 +628,14 @@ static int solve_common(orc_handle h, const double *b, double *x, double tol, in
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     permute_out(h, xp.data(), x);
     fill(st, ms, res);
+    h->last_hist = st.hist;
+    return 0;
+}
+
+int orc_get_history(orc_handle h, double *norms, int32_t capacity, int32_t *count) {
+    const int32_t n = (int32_t)h->last_hist.size();
+    for (int32_t i = 0; i < std::min(n, capacity); ++i) norms[i] = h->last_hist[i];
+    if (count) *count = n;
     return 0;
 }
 

@@ -228,3 +228,23 @@ def test_mixed_edge_cases():
         assert r64.converged and rm.converged and rm.outer_iterations >= 1
         assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-6
     g.close()
+
+
+def test_norm_history(system):
+    """Device-recorded norm history: h + 2 values, ||r0|| first, the final
+    norm last, the norm ring in its last three values; against the oracle's
+    history value by value (dots are summed in different orders: 1e-6)."""
+    name, s, g, o = system
+    g.set_history(4096)
+    for tol, maxit in [(1e-8, 300), (0.0, 4)]:
+        xg, rg = g.solve(s.b, tol=tol, maxit=maxit)
+        hg = g.history()
+        assert len(hg) == rg.half_iterations + 2
+        assert hg[0] == rg.norms[0] and hg[-1] == rg.final_norm
+        ring = [rg.norms[(rg.last_norm_idx - 1 - k) % 3 + 1] for k in range(3)]
+        assert list(hg[::-1][:3]) == ring
+        xo, ro = o.solve(s.b, tol=tol, maxit=maxit)
+        ho = o.history()
+        k = min(len(hg), len(ho), 12)
+        np.testing.assert_allclose(hg[:k], ho[:k], rtol=1e-6)
+    g.set_history(0)

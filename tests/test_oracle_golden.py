@@ -237,3 +237,20 @@ def test_colouring_hashes():
     s = fa.generate(fa.configs.GRID_32x32x16)
     mh = hashlib.sha256(s.rowptr.tobytes() + s.colidx.tobytes() + s.values.tobytes() + s.b.tobytes()).hexdigest()
     assert mh == gold["grid32x32x16"]["matrix_sha256"] and s.nnzb == gold["grid32x32x16"]["nnzb"] == 110592
+
+
+def test_oracle_norm_history_matches_status():
+    """The recorded history starts with ||r0||, ends with the final norm,
+    holds h + 2 values after h half iterations, and its last three values are
+    the norm ring (newest at (h % 3) + 1)."""
+    s = fa.generate(fa.configs.scaled(fa.configs.GRID_32x32x16, 10, 8, 6))
+    o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, s.values)
+    o.factor()
+    for tol, maxit in [(1e-8, 200), (0.0, 3)]:
+        x, r = o.solve(s.b, tol=tol, maxit=maxit)
+        h = o.history()
+        assert len(h) == r["half_iterations"] + 2
+        assert h[0] == r["norms"][0] and h[-1] == r["final_norm"]
+        last = r["last_norm_idx"]
+        ring = [r["norms"][(last - 1 - k) % 3 + 1] for k in range(3)]   # newest first
+        assert list(h[::-1][:3]) == ring
