@@ -157,7 +157,7 @@ struct bda_handle_s {
     bool last_solve_valid = false;
     int last_kind = 0;  // 0 plain / adjoint on k64, 1 mixed (residual in ro)
 
-    // reductions and scalar state (0 = solve, 1 = auxiliary)
+    // reductions and scalar state (0 = solve, 1 = outer loop or auxiliary, 2 = idle)
     bda::DBuf<double> partials;
     int32_t partial_cap = 0;
     bda::DBuf<bda::SolverState> st;
@@ -169,7 +169,6 @@ struct bda_handle_s {
     bda::DBuf<double> hist;             // norm history of the runs on state slot 0 (optional)
     int32_t hist_cap = 0;
     int32_t run_seq = 0;                // last run id handed out
-    hipEvent_t poll_ev[4] = {};
 
     // options
     double timeout_s = 0.0;
@@ -943,7 +942,6 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         HIP_CHECK(hipHostMalloc((void **)&h->h_prog, 2 * sizeof(uint64_t), hipHostMallocMapped));
         h->h_prog[0] = h->h_prog[1] = 0;
         HIP_CHECK(hipHostGetDevicePointer((void **)&h->d_prog, h->h_prog, 0));
-        for (auto &e : h->poll_ev) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         h->d_orig.ensure((size_t)nnzb_max * bs * bs);
     } catch (const std::exception &e) {
         std::fprintf(stderr, "ERROR: bda_create: %s\n", e.what());
@@ -971,8 +969,6 @@ int bda_destroy(bda_handle h) {
     if (h->h_st) (void)hipHostFree(h->h_st);
     if (h->h_done) (void)hipHostFree(h->h_done);
     if (h->h_prog) (void)hipHostFree(h->h_prog);
-    for (auto &e : h->poll_ev)
-        if (e) (void)hipEventDestroy(e);
     for (auto &pr : h->ev_pool) {
         (void)hipEventDestroy(pr.first);
         (void)hipEventDestroy(pr.second);

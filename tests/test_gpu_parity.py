@@ -248,3 +248,21 @@ def test_norm_history(system):
         k = min(len(hg), len(ho), 12)
         np.testing.assert_allclose(hg[:k], ho[:k], rtol=1e-6)
     g.set_history(0)
+
+
+def test_edge_watchdog_abort():
+    """The host watchdog (the counterpart of param0's abort cycles): a tiny
+    timeout aborts the solve, the result says so, and the handle stays
+    usable for the next solve."""
+    s, g, o = _pair(scaled(GRID_32x32x16, 12, 10, 8))
+    g.set_options(timeout_s=1e-9)
+    x, r = g.solve(s.b, tol=1e-14, maxit=1000)
+    assert r.aborted and not r.converged
+    xm, rm = g.solve_mixed(s.b, tol=1e-14)
+    assert rm.aborted and not rm.converged
+    g.set_options(timeout_s=0.0)
+    x, r = g.solve(s.b, tol=1e-8, maxit=300)
+    xo, ro = o.solve(s.b, tol=1e-8, maxit=300)
+    assert r.converged and not r.aborted
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    g.close()
