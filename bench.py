@@ -55,7 +55,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this solver runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md but not built)", flush=True)
+              "(sharding over several GPUs is not built)", flush=True)
         return 2
     import torch
 

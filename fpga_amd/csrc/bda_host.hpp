@@ -1,6 +1,6 @@
 // Host-side data structures shared by the generator, the analysis (colouring,
 // colour-ordered reordering, factor patterns, transposes, tiling) and the
-// device driver.  Plain C++17, no HIP types.
+// host code that runs the device kernels.  Plain C++17, no HIP types.
 #pragma once
 
 #include <cstdint>
