@@ -344,7 +344,7 @@ enum StreamMode : int { SM_SPMV = 0, SM_SUB = 1, SM_SUB_DINV = 2 };
 
 template <typename TV, typename TX>
 struct StreamArgs {
-    const int2 *tile;   // tile descriptors {first row, first block}, ntiles_total + 1
+    const int4 *tile;   // tile descriptors {first row, first block, end row, end block}
     int32_t tile0, ntiles;
     const int32_t *rowptr;
     const int32_t *colidx;
@@ -479,10 +479,10 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     // also reads the descriptor (a negative entry never occurs), so the
     // compiler cannot sink its load below the branch
     const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
-    const int2 d0 = a.tile[tile], d1 = a.tile[tile + 1];
-    if ((a.ra.st->done | ((d0.x | d0.y | d1.x | d1.y) < 0 ? 1 : 0)) != 0) return;
-    const int r0 = d0.x, r1 = d1.x, b0 = d0.y;
-    const int nblk = d1.y - b0;
+    const int4 d = a.tile[tile];
+    if ((a.ra.st->done | ((d.x | d.y | d.z | d.w) < 0 ? 1 : 0)) != 0) return;
+    const int r0 = d.x, r1 = d.z, b0 = d.y;
+    const int nblk = d.w - b0;
     const int nrow = r1 - r0;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;

@@ -59,11 +59,43 @@ struct DevMat {
     int64_t nnzb = 0;
     int ntiles = 0;
     DBuf<int32_t> rowptr, colidx, src;
-    DBuf<int2> tile;    // {first row, first block} per tile, plus the end
-    DBuf<int2> tile32;  // tiles of the FP32-value kernels
+    // tile descriptors {first row, first block, end row, end block}: in
+    // colour order (per-colour launches address a contiguous range), and the
+    // same tiles in spatial order for launches over the whole matrix
+    DBuf<int4> tile, tile_sp;
+    DBuf<int4> tile32, tile32_sp;  // tiles of the FP32-value kernels
     std::vector<int32_t> colour_tile, colour_tile32;
     int ntiles32 = 0;
-    void upload(const MatPattern &p, hipStream_t s) {
+
+    // Spatial order: tiles sorted by the original index of their first row.
+    // A launch over all rows has no ordering constraint, and in this order
+    // the tiles that run at the same time (on one XCD) cover neighbouring
+    // rows of every colour, whose x entries then share that XCD's L2
+    // instead of being fetched again for each colour.
+    static void upload_tiles(const MatPattern &p, const std::vector<int32_t> &tile_row,
+                             const std::vector<int32_t> *orig_row, DBuf<int4> &out, DBuf<int4> &out_sp,
+                             hipStream_t s) {
+        const int nt = (int)tile_row.size() - 1;
+        std::vector<int4> d(std::max(nt, 1), make_int4(0, 0, 0, 0));
+        for (int t = 0; t < nt; ++t)
+            d[t] = make_int4(tile_row[t], p.rowptr[tile_row[t]], tile_row[t + 1], p.rowptr[tile_row[t + 1]]);
+        std::vector<int4> ds = d;
+        if (orig_row && nt > 0) {
+            std::vector<int32_t> ord(nt);
+            for (int t = 0; t < nt; ++t) ord[t] = t;
+            std::stable_sort(ord.begin(), ord.end(),
+                             [&](int a, int b) { return (*orig_row)[tile_row[a]] < (*orig_row)[tile_row[b]]; });
+            for (int t = 0; t < nt; ++t) ds[t] = d[ord[t]];
+        }
+        out.ensure(d.size());
+        out_sp.ensure(d.size());
+        HIP_CHECK(hipMemcpyAsync(out.p, d.data(), d.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(out_sp.p, ds.data(), ds.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+    }
+
+    // orig_row: original row of every (permuted) row, for the spatial order
+    void upload(const MatPattern &p, hipStream_t s, const std::vector<int32_t> *orig_row = nullptr) {
         n = p.n;
         nnzb = p.nnzb();
         ntiles = p.ntiles();
@@ -73,27 +105,21 @@ struct DevMat {
         rowptr.ensure(p.rowptr.size());
         colidx.ensure(p.colidx.size() + kPadBlocks);
         src.ensure(std::max<size_t>(p.src.size(), 1));
-        tile.ensure(p.tile_row.size());
-        std::vector<int2> desc(p.tile_row.size());
-        for (size_t i = 0; i < desc.size(); ++i) desc[i] = make_int2(p.tile_row[i], p.rowptr[p.tile_row[i]]);
-        tile32.ensure(p.tile_row32.size());
-        std::vector<int2> desc32(p.tile_row32.size());
-        for (size_t i = 0; i < desc32.size(); ++i)
-            desc32[i] = make_int2(p.tile_row32[i], p.rowptr[p.tile_row32[i]]);
         HIP_CHECK(hipMemcpyAsync(rowptr.p, p.rowptr.data(), p.rowptr.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.colidx.empty())
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(tile.p, desc.data(), desc.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(tile32.p, desc32.data(), desc32.size() * sizeof(int2), hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipStreamSynchronize(s));
+        upload_tiles(p, p.tile_row, orig_row, tile, tile_sp, s);
+        upload_tiles(p, p.tile_row32, orig_row, tile32, tile32_sp, s);
     }
     void release() {
         rowptr.release();
         colidx.release();
         tile.release();
+        tile_sp.release();
         tile32.release();
+        tile32_sp.release();
         src.release();
     }
 };
@@ -279,7 +305,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     StreamArgs<TV, TX> a;
     constexpr bool f32 = wide_tiles<TV, MODE>();
     const std::vector<int32_t> &ct = f32 ? m.colour_tile32 : m.colour_tile;
-    a.tile = f32 ? m.tile32.p : m.tile.p;
+    a.tile = colour < 0 ? (f32 ? m.tile32_sp.p : m.tile_sp.p) : (f32 ? m.tile32.p : m.tile.p);
     if (colour < 0) {
         a.tile0 = 0;
         a.ntiles = f32 ? m.ntiles32 : m.ntiles;
@@ -314,7 +340,7 @@ void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, 
     StreamArgs<TV, TX> a;
     std::memset(&a, 0, sizeof(a));
     constexpr bool f32 = wide_tiles<TV, SM_SPMV>();
-    a.tile = f32 ? h.A.tile32.p : h.A.tile.p;
+    a.tile = f32 ? h.A.tile32_sp.p : h.A.tile_sp.p;
     a.tile0 = 0;
     a.ntiles = f32 ? h.A.ntiles32 : h.A.ntiles;
     a.rowptr = h.A.rowptr.p;
@@ -576,10 +602,10 @@ void do_analyse(H &h, bool level) {
     if (!h.have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
     analyse(h.pattern, level, h.an);
     const Analysis &an = h.an;
-    h.A.upload(an.A, h.stream);
+    h.A.upload(an.A, h.stream, &an.perm);
     h.L.upload(an.L, h.stream);
     h.U.upload(an.U, h.stream);
-    h.At.upload(an.At, h.stream);
+    h.At.upload(an.At, h.stream, &an.perm);
     h.Ut.upload(an.Ut, h.stream);
     h.Lt.upload(an.Lt, h.stream);
     h.d_perm.ensure(h.n);
