@@ -106,6 +106,7 @@ SIGNATURES = {
     "bda_get_factors": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "bda_set_profiling": (i32, [vp, i32]),
     "bda_get_kernel_stats": (i32, [vp, i32, C.POINTER(i64), P_f64]),
+    "bda_get_kernel_times": (i32, [vp, i32, vp, i32, P_i32]),
     "bda_time_spmv": (i32, [vp, i32, i32, P_f64]),
     "bda_algorithmic_bytes": (i32, [vp, i32, i32, P_f64]),
     "bda_set_history": (i32, [vp, i32]),

@@ -212,6 +212,7 @@ struct bda_handle_s {
     int prof_tag = -1;
     int64_t prof_count[bda::PK_N] = {};
     double prof_ms[bda::PK_N] = {};
+    std::vector<float> prof_times[bda::PK_N];  // every counted launch, ms
 
     size_t N() const { return (size_t)n * bs; }
     size_t Npad() const { return (size_t)n * bs + bda::kVecPad; }
@@ -258,6 +259,7 @@ void prof_collect(H &h, int max_tag = INT_MAX) {
         HIP_CHECK(hipEventElapsedTime(&ms, h.ev_pool[i].first, h.ev_pool[i].second));
         h.prof_count[h.ev_kind[i]]++;
         h.prof_ms[h.ev_kind[i]] += ms;
+        h.prof_times[h.ev_kind[i]].push_back(ms);
     }
     h.ev_used = 0;
 }
@@ -297,10 +299,12 @@ int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
 }
 
 // --- streaming kernel launch over the tiles of one colour (or all) ---------
+// prof_kind >= 0: with profiling on, the streaming kernel alone (not its
+// finalize) is bracketed by an event pair of that kind.
 template <int BS, int MODE, int RED, typename TV, typename TX>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
                    const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
-                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0) {
+                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
     constexpr bool f32 = wide_tiles<TV, MODE>();
@@ -331,7 +335,9 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
     if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
+    if (prof_kind >= 0) prof_begin(h, prof_kind);
     const int nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+    if (prof_kind >= 0) prof_end(h);
     if (nred > 0) launch_finalize(h, nred, a.ra, nparts);
 }
 
@@ -483,10 +489,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         launch_finalize(h, 1, red(h, 0, FIN_INIT), vb);
     } else {
         if constexpr (std::is_same<TX, double>::value) {
-            prof_begin(h, PK_SPMV);
             launch_stream<BS, SM_SUB, SR_SELF, TV, TX>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr, nullptr,
-                                                       red(h, 0, FIN_INIT), true, k.r.p, k.p.p);
-            prof_end(h);
+                                                       red(h, 0, FIN_INIT), true, k.r.p, k.p.p, nullptr, 0, PK_SPMV);
         } else {
             throw std::runtime_error("non-zero start vector needs FP64 vectors");
         }
@@ -501,10 +505,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     for (int it = 1; it <= maxit + 1 && !stop; ++it) {
         h.prof_tag = 2 * (it - 1);
         precond<BS, TV, TX>(h, op, k.p.p, k.y.p, k.phat.p, true);
-        prof_begin(h, PK_SPMV);
         launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p,
-                                                  red(h, 0, FIN_ALPHA), true);
-        prof_end(h);
+                                                  red(h, 0, FIN_ALPHA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
         hipLaunchKernelGGL((k_update_xs<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
                            k.phat.p, k.r.p, k.v.p, k.s.p, red(h, 0, FIN_HALF1));
@@ -513,10 +515,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         prof_end(h);
         h.prof_tag = 2 * (it - 1) + 1;
         precond<BS, TV, TX>(h, op, k.s.p, k.y.p, k.shat.p, true);
-        prof_begin(h, PK_SPMV);
         launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p,
-                                                    red(h, 0, FIN_OMEGA), true);
-        prof_end(h);
+                                                    red(h, 0, FIN_OMEGA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
         hipLaunchKernelGGL((k_update_xr<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
                            k.shat.p, k.s.p, k.t.p, k.r.p, k.rt.p, red(h, 0, FIN_HALF2));
@@ -787,10 +787,8 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     bool aborted = false;
     for (int outer = 0;; ++outer) {
         // FP64 true residual ro = b - A x and the outer step's decisions
-        prof_begin(h, PK_SPMV);
         launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
                                                            nullptr, red(h, 1, FIN_REFINE), true);
-        prof_end(h);
         if (outer == max_outer) break;
         if (!wait_progress(h, 1, os.run_id, outer + 1, t0)) {
             aborted = true;
@@ -1305,6 +1303,7 @@ int bda_set_profiling(bda_handle h, int32_t on) {
         for (int k = 0; k < PK_N; ++k) {
             h->prof_count[k] = 0;
             h->prof_ms[k] = 0.0;
+            h->prof_times[k].clear();
         }
         h->ev_used = 0;
     });
@@ -1315,6 +1314,17 @@ int bda_get_kernel_stats(bda_handle h, int32_t kind, int64_t *launches, double *
         if (kind < 0 || kind >= PK_N) throw std::runtime_error("bad kernel kind");
         if (launches) *launches = h->prof_count[kind];
         if (total_ms) *total_ms = h->prof_ms[kind];
+    });
+}
+
+int bda_get_kernel_times(bda_handle h, int32_t kind, float *ms, int32_t capacity, int32_t *count) {
+    return guarded(h, [&] {
+        if (kind < 0 || kind >= PK_N) throw std::runtime_error("bad kernel kind");
+        if (capacity < 0) throw std::runtime_error("negative capacity");
+        const std::vector<float> &v = h->prof_times[kind];
+        const int32_t n = (int32_t)std::min<size_t>(v.size(), (size_t)capacity);
+        if (n > 0 && ms) std::memcpy(ms, v.data(), (size_t)n * sizeof(float));
+        if (count) *count = (int32_t)v.size();
     });
 }
 

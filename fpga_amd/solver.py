@@ -332,6 +332,14 @@ class BicgstabSolver:
         self._check(lib.bda_get_kernel_stats(self._h, kind, C.byref(n), C.byref(ms)), "bda_get_kernel_stats")
         return n.value, ms.value
 
+    def kernel_times(self, kind: int, capacity: int = 1 << 16) -> np.ndarray:
+        """Durations (ms) of every launch of one kind counted since profiling was switched on."""
+        out = np.empty(max(capacity, 1), dtype=np.float32)
+        n = C.c_int32()
+        self._check(lib.bda_get_kernel_times(self._h, kind, out.ctypes.data_as(C.c_void_p), capacity, C.byref(n)),
+                    "bda_get_kernel_times")
+        return out[:min(n.value, capacity)].astype(np.float64)
+
     def set_history(self, capacity: int):
         """Record ||r0|| and every half-iteration's residual norm (up to capacity values)."""
         self._check(lib.bda_set_history(self._h, capacity), "bda_set_history")

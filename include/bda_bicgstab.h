@@ -303,6 +303,12 @@ int bda_wire_import(int32_t bs, const int32_t vector_sizes[18], void *const vect
  * updates.  Counts and summed milliseconds since the last reset. */
 int bda_set_profiling(bda_handle h, int32_t on);
 int bda_get_kernel_stats(bda_handle h, int32_t kind, int64_t *launches, double *total_ms);
+/* The individual durations (ms) of the launches counted above, in launch
+ * order: up to capacity values go to ms, *count receives how many there
+ * are.  Kind 0 brackets the streaming SpMV kernel alone (not the reduction
+ * that follows it).  A diagnostic of this build; no counterpart in the
+ * reference. */
+int bda_get_kernel_times(bda_handle h, int32_t kind, float *ms, int32_t capacity, int32_t *count);
 /* reps back-to-back launches of the solve's SpMV kernel (v = A phat with
  * the fused v.rt partial sums; FP64, or FP32 values and vectors with
  * fp32 != 0) on device-resident vectors; average launch time in ms from two
