@@ -113,7 +113,7 @@ void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, in
 
 void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) {
     build_tiles(p, colour_ptr, tile_cap(bs, false), p.tile_row, p.colour_tile);
-    build_tiles(p, colour_ptr, tile_cap(bs, true), p.tile_row32, p.colour_tile32);
+    build_tiles(p, colour_ptr, tile_cap(bs, true), p.tile_row_wide, p.colour_tile_wide);
 }
 
 void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {

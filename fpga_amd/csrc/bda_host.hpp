@@ -11,11 +11,12 @@ namespace bda {
 
 // Streaming kernels process "tiles": runs of consecutive block rows that
 // belong to one colour, hold at most kTileRows rows and at most
-// tile_cap(bs, fp32) blocks, so that one workgroup can hold the tile's
-// values in registers and stage its per-block partial sums in LDS.  FP32
-// values take half the registers, so their tiles hold twice the blocks.
+// tile_cap(bs, wide) blocks, so that one workgroup can hold the tile's
+// values in registers and stage its per-block partial sums in LDS.  Every
+// pattern has a narrow tiling (the sweeps) and a wide one with twice the
+// blocks per tile (the SpMV).
 constexpr int kTileRows = 64;
-inline int tile_cap(int bs, bool fp32 = false) { return (bs == 3 ? 168 : 128) * (fp32 ? 2 : 1); }
+inline int tile_cap(int bs, bool wide = false) { return (bs == 3 ? 168 : 128) * (wide ? 2 : 1); }
 
 struct Bcsr {
     int bs = 0;
@@ -33,8 +34,8 @@ struct MatPattern {
     std::vector<int32_t> src;        // source block index (meaning depends on the matrix)
     std::vector<int32_t> tile_row;   // tile boundaries (row indices), size ntiles+1
     std::vector<int32_t> colour_tile;// first tile of each colour, size ncolours+1
-    std::vector<int32_t> tile_row32; // the same for the FP32 kernels' larger tiles
-    std::vector<int32_t> colour_tile32;
+    std::vector<int32_t> tile_row_wide;  // the same for the wide tiling
+    std::vector<int32_t> colour_tile_wide;
     int nnzb() const { return rowptr.empty() ? 0 : rowptr.back(); }
     int ntiles() const { return tile_row.empty() ? 0 : (int)tile_row.size() - 1; }
 };

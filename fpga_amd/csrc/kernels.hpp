@@ -407,29 +407,26 @@ constexpr int kPadBlocks = 8;
 // FP32 3x3, 16 for FP32 4x4; 24 and 32 bytes for FP64).
 template <typename T, int BS> struct __attribute__((packed, aligned(sizeof(T)))) XVec { T v[BS]; };
 
-// Tile capacity in blocks; the host tiler uses the same numbers.
-template <int BS, typename TV> struct TileCap;
-template <> struct TileCap<3, double> { static constexpr int value = 168; };  // 8 workgroup steps of 21 blocks
-template <> struct TileCap<4, double> { static constexpr int value = 128; };  // 8 workgroup steps of 16 blocks
-template <> struct TileCap<3, float> { static constexpr int value = 336; };   // FP32 values: 16 steps
-template <> struct TileCap<4, float> { static constexpr int value = 256; };
+// Tile capacity in blocks, narrow and wide tiling; the host tiler uses the
+// same numbers.
+template <int BS, bool WIDE> struct TileCap;
+template <> struct TileCap<3, false> { static constexpr int value = 168; };  // 8 workgroup steps of 21 blocks
+template <> struct TileCap<4, false> { static constexpr int value = 128; };  // 8 workgroup steps of 16 blocks
+template <> struct TileCap<3, true> { static constexpr int value = 336; };   // 16 steps
+template <> struct TileCap<4, true> { static constexpr int value = 256; };
 
-// Which tiling a kernel instance uses: the wide FP32 tiling for FP32-value
-// kernels in the modes whose bit is set in BDA_WIDE_MODES, the FP64 tiling
-// otherwise.
+// Which tiling a kernel instance uses: the wide one in the modes whose bit is
+// set in BDA_WIDE_MODES (the SpMV, in both precisions), the narrow one
+// otherwise (the sweeps: with wide tiles their register and LDS use cuts the
+// resident workgroups, and the FP32 sweeps lose more than they gain).
 #ifndef BDA_WIDE_MODES
 #define BDA_WIDE_MODES 1
 #endif
-template <typename TV, int MODE> constexpr bool wide_tiles() {
-    return std::is_same<TV, float>::value && ((BDA_WIDE_MODES >> MODE) & 1);
-}
-template <int BS, typename TV, int MODE> constexpr int stream_cap() {
-    return wide_tiles<TV, MODE>() ? TileCap<BS, float>::value : TileCap<BS, double>::value;
-}
+template <int MODE> constexpr bool wide_tiles() { return ((BDA_WIDE_MODES >> MODE) & 1) != 0; }
+template <int BS, int MODE> constexpr int stream_cap() { return TileCap<BS, wide_tiles<MODE>()>::value; }
 
 // Matrix values and column indices are read once per launch and use
-// non-temporal loads.  (The x entries are still fetched into L2 three to four
-// times per launch: the measured fetch volume is the same with default,
+// non-temporal loads.  (The measured fetch volume is the same with default,
 // non-temporal and system-scope value loads.)
 //
 // Reductions fused into the epilogue: bit 0 = sum out*w0, bit 1 = sum out*out.
@@ -462,7 +459,7 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
-    constexpr int CAP = stream_cap<BS, TV, MODE>();
+    constexpr int CAP = stream_cap<BS, MODE>();
     constexpr int BPW = 64 / BB;           // blocks per wave step (7 or 4)
     constexpr int VPW = BPW * BB;          // lanes carrying a value (63 or 64)
     constexpr int S = CAP / (NW * BPW);    // value slots per lane

@@ -63,9 +63,9 @@ struct DevMat {
     // colour order (per-colour launches address a contiguous range), and the
     // same tiles in spatial order for launches over the whole matrix
     DBuf<int4> tile, tile_sp;
-    DBuf<int4> tile32, tile32_sp;  // tiles of the FP32-value kernels
-    std::vector<int32_t> colour_tile, colour_tile32;
-    int ntiles32 = 0;
+    DBuf<int4> tile_wide, tile_wide_sp;  // the wide tiling
+    std::vector<int32_t> colour_tile, colour_tile_wide;
+    int ntiles_wide = 0;
 
     // Spatial order: tiles sorted by the original index of their first row.
     // A launch over all rows has no ordering constraint, and in this order
@@ -100,8 +100,8 @@ struct DevMat {
         nnzb = p.nnzb();
         ntiles = p.ntiles();
         colour_tile = p.colour_tile;
-        colour_tile32 = p.colour_tile32;
-        ntiles32 = (int)p.tile_row32.size() - 1;
+        colour_tile_wide = p.colour_tile_wide;
+        ntiles_wide = (int)p.tile_row_wide.size() - 1;
         rowptr.ensure(p.rowptr.size());
         colidx.ensure(p.colidx.size() + kPadBlocks);
         src.ensure(std::max<size_t>(p.src.size(), 1));
@@ -111,15 +111,15 @@ struct DevMat {
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
         upload_tiles(p, p.tile_row, orig_row, tile, tile_sp, s);
-        upload_tiles(p, p.tile_row32, orig_row, tile32, tile32_sp, s);
+        upload_tiles(p, p.tile_row_wide, orig_row, tile_wide, tile_wide_sp, s);
     }
     void release() {
         rowptr.release();
         colidx.release();
         tile.release();
         tile_sp.release();
-        tile32.release();
-        tile32_sp.release();
+        tile_wide.release();
+        tile_wide_sp.release();
         src.release();
     }
 };
@@ -221,8 +221,8 @@ struct bda_handle_s {
 namespace bda {
 namespace {
 
-static_assert(TileCap<3, double>::value == 168 && TileCap<4, double>::value == 128 &&
-                  TileCap<3, float>::value == 336 && TileCap<4, float>::value == 256,
+static_assert(TileCap<3, false>::value == 168 && TileCap<4, false>::value == 128 &&
+                  TileCap<3, true>::value == 336 && TileCap<4, true>::value == 256,
               "device tile capacity must match tile_cap()");
 
 using H = bda_handle_s;
@@ -307,12 +307,12 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
                    TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
-    constexpr bool f32 = wide_tiles<TV, MODE>();
-    const std::vector<int32_t> &ct = f32 ? m.colour_tile32 : m.colour_tile;
-    a.tile = colour < 0 ? (f32 ? m.tile32_sp.p : m.tile_sp.p) : (f32 ? m.tile32.p : m.tile.p);
+    constexpr bool wide = wide_tiles<MODE>();
+    const std::vector<int32_t> &ct = wide ? m.colour_tile_wide : m.colour_tile;
+    a.tile = colour < 0 ? (wide ? m.tile_wide_sp.p : m.tile_sp.p) : (wide ? m.tile_wide.p : m.tile.p);
     if (colour < 0) {
         a.tile0 = 0;
-        a.ntiles = f32 ? m.ntiles32 : m.ntiles;
+        a.ntiles = wide ? m.ntiles_wide : m.ntiles;
     } else {
         a.tile0 = ct[colour];
         a.ntiles = ct[colour + 1] - ct[colour];
@@ -345,10 +345,10 @@ template <int BS, typename TV, typename TX>
 void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, RedArgs ra) {
     StreamArgs<TV, TX> a;
     std::memset(&a, 0, sizeof(a));
-    constexpr bool f32 = wide_tiles<TV, SM_SPMV>();
-    a.tile = f32 ? h.A.tile32_sp.p : h.A.tile_sp.p;
+    constexpr bool wide = wide_tiles<SM_SPMV>();
+    a.tile = wide ? h.A.tile_wide_sp.p : h.A.tile_sp.p;
     a.tile0 = 0;
-    a.ntiles = f32 ? h.A.ntiles32 : h.A.ntiles;
+    a.ntiles = wide ? h.A.ntiles_wide : h.A.ntiles;
     a.rowptr = h.A.rowptr.p;
     a.colidx = h.A.colidx.p;
     a.val = val;
