@@ -83,7 +83,7 @@ int colour_rows(const Bcsr &m, bool level_scheduling, std::vector<int32_t> &colo
 }
 
 void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, int cap,
-                 std::vector<int32_t> &tile_row, std::vector<int32_t> &colour_tile) {
+                 std::vector<int32_t> &tile_row, std::vector<int32_t> &colour_tile, int max_rows) {
     const int ncol = (int)colour_ptr.size() - 1;
     tile_row.clear();
     colour_tile.assign(ncol + 1, 0);
@@ -94,7 +94,7 @@ void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, in
         while (r < end) {
             tile_row.push_back(r);
             int rows = 0, blocks = 0;
-            while (r < end && rows < kTileRows) {
+            while (r < end && rows < max_rows) {
                 const int nb = p.rowptr[r + 1] - p.rowptr[r];
                 if (nb > cap)
                     throw std::runtime_error("block row " + std::to_string(r) + " has " + std::to_string(nb) +
@@ -114,6 +114,7 @@ void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, in
 void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) {
     build_tiles(p, colour_ptr, tile_cap(bs, false), p.tile_row, p.colour_tile);
     build_tiles(p, colour_ptr, tile_cap(bs, true), p.tile_row_wide, p.colour_tile_wide);
+    build_tiles(p, colour_ptr, tile_cap(bs, false), p.tile_row_tall, p.colour_tile_tall, kTileRowsTall);
 }
 
 void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {

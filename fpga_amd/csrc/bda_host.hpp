@@ -16,6 +16,9 @@ namespace bda {
 // pattern has a narrow tiling (the sweeps) and a wide one with twice the
 // blocks per tile (the SpMV).
 constexpr int kTileRows = 64;
+// A third, tall tiling (narrow block capacity, up to 128 rows) serves sweep
+// colours with few blocks per row, whose narrow tiles are short of work.
+constexpr int kTileRowsTall = 128;
 inline int tile_cap(int bs, bool wide = false) { return (bs == 3 ? 168 : 128) * (wide ? 2 : 1); }
 
 struct Bcsr {
@@ -36,6 +39,8 @@ struct MatPattern {
     std::vector<int32_t> colour_tile;// first tile of each colour, size ncolours+1
     std::vector<int32_t> tile_row_wide;  // the same for the wide tiling
     std::vector<int32_t> colour_tile_wide;
+    std::vector<int32_t> tile_row_tall;  // and for the tall tiling
+    std::vector<int32_t> colour_tile_tall;
     int nnzb() const { return rowptr.empty() ? 0 : rowptr.back(); }
     int ntiles() const { return tile_row.empty() ? 0 : (int)tile_row.size() - 1; }
 };
@@ -70,7 +75,7 @@ void analyse(const Bcsr &m, bool level_scheduling, Analysis &an);
 // Build tiles for pattern p whose rows are grouped by colour_ptr.
 void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs);
 void build_tiles(const MatPattern &p, const std::vector<int32_t> &colour_ptr, int cap,
-                 std::vector<int32_t> &tile_row, std::vector<int32_t> &colour_tile);
+                 std::vector<int32_t> &tile_row, std::vector<int32_t> &colour_tile, int max_rows = kTileRows);
 
 // Synthetic grid --------------------------------------------------------
 struct GridSpec {

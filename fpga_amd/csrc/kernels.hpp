@@ -454,7 +454,10 @@ template <int BS, int MODE, typename TV, typename TX> struct StreamWaves {
 // pointers are issued first, so the first barrier waits for them and not
 // for the value stream issued after them.  A tile costs three dependent
 // memory round trips: descriptor; columns/values/row operands; x gathers.
-template <int BS, int MODE, int RED, typename TV, typename TX>
+//
+// ROWS = 128 (the tall tiling, used by sweep colours with few blocks per row)
+// gives every thread two rows of the epilogue, 64 rows apart.
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
 __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamWaves<BS, MODE, TV, TX>::value))) void k_stream(StreamArgs<TV, TX> a) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
@@ -466,6 +469,9 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     constexpr int CJ = (CAP + NT - 1) / NT;       // column loads per thread
     constexpr int XV = (CAP + NT - 1) / NT;       // x gathers (whole blocks) per thread
     constexpr int NRED = (RED & 1) + ((RED >> 1) & 1);
+    constexpr int RS = ROWS / 64;                 // epilogue rows per thread
+    static_assert(ROWS == 64 || ROWS == 128, "tile rows");
+    static_assert(ROWS + 1 <= NT, "one row pointer per thread");
 
     // LDS arrays are sized to whole multiples of the workgroup so that their
     // fills are unconditional (a conditional store lets the compiler sink the
@@ -474,7 +480,7 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     __shared__ TX sx[CAP * BS];
     __shared__ int32_t scol[CJ * NT];
     __shared__ int32_t srow[NT];
-    __shared__ double tv[MODE == SM_SUB_DINV ? NT : 1];
+    __shared__ double tv[MODE == SM_SUB_DINV ? RS * NT : 1];
 
     // the stop flag and the tile descriptor are loaded together (one scalar
     // round trip instead of two before the first vector load): the exit test
@@ -506,19 +512,28 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
         av[u] = load_stream(&val[(size_t)min(bl, nblk) * BB]);
     }
 
-    // 3. per-row operands of the epilogue
-    const int k = t / BS, r = t - k * BS;
-    const int row = r0 + k;
-    const bool has_row = k < nrow;
-    const size_t gi = (size_t)(r0 + min(k, nrow - 1)) * BS + r;
-    double rhs_v = 0.0, w_v = 0.0;
-    double dv[BS];
-    if constexpr (MODE != SM_SPMV) rhs_v = (double)a.rhs[gi];
-    if constexpr ((RED & SR_W0) != 0) w_v = (double)a.w0[gi];
-    if constexpr (MODE == SM_SUB_DINV) {
-        const double *D = a.dinv + gi * BS;
+    // 3. per-row operands of the epilogue: thread t serves scalar row r of
+    // block rows k0 and (ROWS = 128) k0 + 64
+    const int k0 = t / BS, r = t - k0 * BS;
+    int kk[RS];
+    bool has_row[RS];
+    size_t gi[RS];
+    double rhs_v[RS], w_v[RS];
+    double dv[RS][BS];
 #pragma unroll
-        for (int c = 0; c < BS; ++c) dv[c] = D[c];
+    for (int s = 0; s < RS; ++s) {
+        kk[s] = k0 + 64 * s;
+        has_row[s] = kk[s] < nrow;
+        gi[s] = (size_t)(r0 + min(kk[s], nrow - 1)) * BS + r;
+        rhs_v[s] = 0.0;
+        w_v[s] = 0.0;
+        if constexpr (MODE != SM_SPMV) rhs_v[s] = (double)a.rhs[gi[s]];
+        if constexpr ((RED & SR_W0) != 0) w_v[s] = (double)a.w0[gi[s]];
+        if constexpr (MODE == SM_SUB_DINV) {
+            const double *D = a.dinv + gi[s] * BS;
+#pragma unroll
+            for (int c = 0; c < BS; ++c) dv[s][c] = D[c];
+        }
     }
 
 #pragma unroll
@@ -568,37 +583,50 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     }
     __syncthreads();
 
-    double res = 0.0;
-    if (has_row) {
-        const int q1 = srow[k], q2 = srow[k + 1];
-        double sum = 0.0;
-        for (int q = q1; q < q2; ++q) sum += part[q * BS + r];
-        if constexpr (MODE == SM_SPMV) res = sum;
-        else res = rhs_v - sum;
+    double res[RS];
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+        res[s] = 0.0;
+        if (has_row[s]) {
+            const int q1 = srow[kk[s]], q2 = srow[kk[s] + 1];
+            double sum = 0.0;
+            for (int q = q1; q < q2; ++q) sum += part[q * BS + r];
+            if constexpr (MODE == SM_SPMV) res[s] = sum;
+            else res[s] = rhs_v[s] - sum;
+        }
     }
     if constexpr (MODE == SM_SUB_DINV) {
-        tv[t] = res;
-        __syncthreads();
-        if (has_row) {
-            double z = 0.0;
 #pragma unroll
-            for (int c = 0; c < BS; ++c) z += dv[c] * tv[k * BS + c];
-            res = z;
+        for (int s = 0; s < RS; ++s) tv[s * NT + t] = res[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+            if (has_row[s]) {
+                double z = 0.0;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) z += dv[s][c] * tv[s * NT + k0 * BS + c];
+                res[s] = z;
+            }
         }
     }
     double red[2] = {0.0, 0.0};
-    if (has_row) {
-        const TX o = from_double<TX>(res);
-        a.out[gi] = o;
-        if (a.out2) a.out2[gi] = o;
-        if (a.out3) a.out3[gi] = o;
+#pragma unroll
+    for (int s = 0; s < RS; ++s) {
+        if (!has_row[s]) continue;
+        const TX o = from_double<TX>(res[s]);
+        a.out[gi[s]] = o;
+        if (a.out2) a.out2[gi[s]] = o;
+        if (a.out3) a.out3[gi[s]] = o;
         const double od = (double)o;
-        if constexpr (RED == SR_W0) red[0] = od * w_v;
-        if constexpr (RED == SR_SELF) red[0] = od * od;
+        double v0 = 0.0, v1 = 0.0;
+        if constexpr (RED == SR_W0) v0 = od * w_v[s];
+        if constexpr (RED == SR_SELF) v0 = od * od;
         if constexpr (RED == SR_BOTH) {
-            red[0] = od * w_v;
-            red[1] = od * od;
+            v0 = od * w_v[s];
+            v1 = od * od;
         }
+        red[0] = s == 0 ? v0 : red[0] + v0;
+        red[1] = s == 0 ? v1 : red[1] + v1;
     }
     if constexpr (NRED == 1) {
         double v1[1] = {red[0]};

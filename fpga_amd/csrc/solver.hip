@@ -64,7 +64,8 @@ struct DevMat {
     // same tiles in spatial order for launches over the whole matrix
     DBuf<int4> tile, tile_sp;
     DBuf<int4> tile_wide, tile_wide_sp;  // the wide tiling
-    std::vector<int32_t> colour_tile, colour_tile_wide;
+    DBuf<int4> tile_tall;                // the tall tiling (sweeps, colour order only)
+    std::vector<int32_t> colour_tile, colour_tile_wide, colour_tile_tall;
     int ntiles_wide = 0;
 
     // Spatial order: tiles sorted by the original index of their first row.
@@ -73,7 +74,7 @@ struct DevMat {
     // rows of every colour, whose x entries then share that XCD's L2
     // instead of being fetched again for each colour.
     static void upload_tiles(const MatPattern &p, const std::vector<int32_t> &tile_row,
-                             const std::vector<int32_t> *orig_row, DBuf<int4> &out, DBuf<int4> &out_sp,
+                             const std::vector<int32_t> *orig_row, DBuf<int4> &out, DBuf<int4> *out_sp,
                              hipStream_t s) {
         const int nt = (int)tile_row.size() - 1;
         std::vector<int4> d(std::max(nt, 1), make_int4(0, 0, 0, 0));
@@ -88,9 +89,11 @@ struct DevMat {
             for (int t = 0; t < nt; ++t) ds[t] = d[ord[t]];
         }
         out.ensure(d.size());
-        out_sp.ensure(d.size());
         HIP_CHECK(hipMemcpyAsync(out.p, d.data(), d.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-        HIP_CHECK(hipMemcpyAsync(out_sp.p, ds.data(), ds.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        if (out_sp) {
+            out_sp->ensure(d.size());
+            HIP_CHECK(hipMemcpyAsync(out_sp->p, ds.data(), ds.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        }
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
@@ -101,6 +104,7 @@ struct DevMat {
         ntiles = p.ntiles();
         colour_tile = p.colour_tile;
         colour_tile_wide = p.colour_tile_wide;
+        colour_tile_tall = p.colour_tile_tall;
         ntiles_wide = (int)p.tile_row_wide.size() - 1;
         rowptr.ensure(p.rowptr.size());
         colidx.ensure(p.colidx.size() + kPadBlocks);
@@ -110,8 +114,9 @@ struct DevMat {
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
-        upload_tiles(p, p.tile_row, orig_row, tile, tile_sp, s);
-        upload_tiles(p, p.tile_row_wide, orig_row, tile_wide, tile_wide_sp, s);
+        upload_tiles(p, p.tile_row, orig_row, tile, &tile_sp, s);
+        upload_tiles(p, p.tile_row_wide, orig_row, tile_wide, &tile_wide_sp, s);
+        upload_tiles(p, p.tile_row_tall, nullptr, tile_tall, nullptr, s);
     }
     void release() {
         rowptr.release();
@@ -120,6 +125,7 @@ struct DevMat {
         tile_sp.release();
         tile_wide.release();
         tile_wide_sp.release();
+        tile_tall.release();
         src.release();
     }
 };
@@ -291,9 +297,9 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
 
 // Launches the streaming kernel, one workgroup per tile; returns the number
 // of reduction partials written.
-template <int BS, int MODE, int RED, typename TV, typename TX>
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
 int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
-    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX, ROWS>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
     check_launch();
     return a.ntiles;
 }
@@ -318,6 +324,20 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
         a.ntiles = ct[colour + 1] - ct[colour];
     }
     if (a.ntiles <= 0) return;
+    // a sweep colour with few blocks per row takes the tall tiling when that
+    // saves a quarter of the workgroups or more
+    bool tall = false;
+    if constexpr (MODE != SM_SPMV && RED == SR_NONE) {
+        if (colour >= 0 && !wide) {
+            const int nt = m.colour_tile_tall[colour + 1] - m.colour_tile_tall[colour];
+            if (4 * nt <= 3 * a.ntiles) {
+                tall = true;
+                a.tile = m.tile_tall.p;
+                a.tile0 = m.colour_tile_tall[colour];
+                a.ntiles = nt;
+            }
+        }
+    }
     a.rowptr = m.rowptr.p;
     a.colidx = m.colidx.p;
     a.val = val;
@@ -336,7 +356,13 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
     if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
     if (prof_kind >= 0) prof_begin(h, prof_kind);
-    const int nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+    int nparts = 0;
+    if constexpr (MODE != SM_SPMV && RED == SR_NONE) {
+        if (tall) nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRowsTall>(h, a);
+        else nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+    } else {
+        nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+    }
     if (prof_kind >= 0) prof_end(h);
     if (nred > 0) launch_finalize(h, nred, a.ra, nparts);
 }
