@@ -1,0 +1,98 @@
+"""GPU parity on a matrix built so that the sweeps take the tall tiling.
+
+Most block rows are uncoupled; a few couple to rows far away, which gives
+three colours whose rows carry at most one or two lower or upper blocks.
+With 64-row tiles such a sweep colour needs twice the workgroups of the tall
+(128-row) tiling, so every sweep launch of the preconditioner except the
+last colour's takes the 128-row kernel: the lower sweep of colour 1, and the
+upper sweeps of colours 1 and 0.  The results must stay bit-identical to the
+oracle, as for the other tilings.
+"""
+import numpy as np
+import pytest
+
+import fpga_amd as fa
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def sparse_coupled(bs, n=2000, seed=0):
+    """Block rows i < 200 couple to 1000 + i, and 1000 + i (i < 50) to 1500 + i,
+    which also couples to i; rows 200 + i (i < 40) couple to 1000 + i and
+    1001 + i, so that ILU0 drops fill.  Greedy colouring in natural order gives
+    colour 0 to 1,750 rows, colour 1 to rows 1000..1199 and colour 2 to
+    1500..1549."""
+    rng = np.random.default_rng(seed)
+    nb = [set([i]) for i in range(n)]
+    for i in range(200):
+        nb[i].add(1000 + i); nb[1000 + i].add(i)
+    for i in range(50):
+        nb[1000 + i].add(1500 + i); nb[1500 + i].add(1000 + i)
+        nb[i].add(1500 + i); nb[1500 + i].add(i)
+    for i in range(40):  # fill that ILU0 drops (between rows 1000 + i and 1001 + i)
+        for j in (1000 + i, 1001 + i):
+            nb[200 + i].add(j); nb[j].add(200 + i)
+    rowptr = np.zeros(n + 1, np.int32)
+    cols = []
+    for i in range(n):
+        c = sorted(nb[i])
+        cols.extend(c)
+        rowptr[i + 1] = rowptr[i] + len(c)
+    colidx = np.array(cols, np.int32)
+    vals = rng.uniform(-0.5, 0.5, (len(cols), bs, bs))
+    for i in range(n):
+        for q in range(rowptr[i], rowptr[i + 1]):
+            if colidx[q] == i:
+                off = sum(np.abs(vals[p]).sum(axis=1) for p in range(rowptr[i], rowptr[i + 1]) if colidx[p] != i)
+                off = off if np.ndim(off) else np.zeros(bs)
+                vals[q] = np.diag(off + 1.0) + rng.normal(0.0, 0.05, (bs, bs))
+    b = rng.uniform(-1, 1, n * bs)
+    return rowptr, colidx, vals, b
+
+
+@pytest.fixture(scope="module", params=[3, 4])
+def pair(request):
+    bs = request.param
+    print(f"case sparse_coupled bs={bs}", flush=True)
+    rowptr, colidx, vals, b = sparse_coupled(bs)
+    g = fa.BicgstabSolver(bs, len(rowptr) - 1, len(colidx))
+    g.upload_bcsr(rowptr, colidx, vals)
+    g.analyse()
+    g.factor()
+    o = oracle.OracleSystem(bs, rowptr, colidx, vals)
+    o.factor()
+    yield bs, b, g, o
+    g.close()
+
+
+def test_colours_as_built(pair):
+    bs, b, g, o = pair
+    colour, perm, cptr = g.colouring()
+    assert g.num_colours == 3
+    np.testing.assert_array_equal(np.diff(cptr), [1750, 200, 50])
+
+
+def test_precond_bit_exact_tall_tiles(pair):
+    bs, b, g, o = pair
+    p = np.random.default_rng(5).uniform(-1, 1, len(b))
+    np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
+    np.testing.assert_array_equal(g.precond_apply_perm(p, transpose=True), o.precond_perm(p, transpose=True))
+
+
+def test_spmv_bit_exact(pair):
+    bs, b, g, o = pair
+    x = np.random.default_rng(6).uniform(-1, 1, len(b))
+    np.testing.assert_array_equal(g.spmv_perm(x), o.spmv_perm(x))
+
+
+def test_solve_parity(pair):
+    bs, b, g, o = pair
+    xg, rg = g.solve(b, tol=1e-10, maxit=200)
+    xo, ro = o.solve(b, tol=1e-10, maxit=200)
+    assert rg.converged and ro["converged"]
+    assert abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
+    xm, rm = g.solve_mixed(b, tol=1e-8)
+    assert rm.converged
+    np.testing.assert_allclose(xm, xo, rtol=0, atol=1e-6 * np.abs(xo).max())
