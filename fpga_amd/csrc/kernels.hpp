@@ -327,16 +327,17 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts
 // ---------------------------------------------------------------------------
 // Streaming block-row kernel.
 //
-// One workgroup (64*BS threads, BS waves) handles one tile: up to 64
-// consecutive block rows of one colour whose blocks are contiguous in the
-// value array.  Phase 1 streams the tile's values with one lane per matrix
-// entry (7 blocks of 3x3 = 63 lanes, or 4 blocks of 4x4 = 64 lanes per wave
-// step), so every wave load instruction reads consecutive memory; each lane
-// gathers its x entry (column indices staged in LDS), multiplies, and the
-// BS lanes of one block row combine their products left to right with
+// One workgroup (64*BS threads, BS waves) handles one tile: up to 64 (or,
+// in the tall tiling, 128) consecutive block rows of one colour whose blocks
+// are contiguous in the value array.  Phase 1 streams the tile's values with
+// one lane per matrix entry (7 blocks of 3x3 = 63 lanes, or 4 blocks of 4x4
+// = 64 lanes per wave step), so every wave load instruction reads
+// consecutive memory; the tile's x entries are gathered into LDS (column
+// indices staged there first), each lane multiplies its entry, and the BS
+// lanes of one block row combine their products left to right with
 // cross-lane shuffles.  The per-block row partials go to LDS.  Phase 2 gives
-// one thread per scalar row, which adds its block partials in column order
-// and applies the epilogue.
+// one thread per scalar row (two in the tall tiling), which adds its block
+// partials in column order and applies the epilogue.
 //
 // Summation order per scalar row r of block row i:
 //   sum = 0;  for blocks j in column order: sum += ((a_r0 x_0 + a_r1 x_1) + a_r2 x_2)
