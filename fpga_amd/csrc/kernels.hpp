@@ -105,8 +105,8 @@ __device__ inline void signal_done(SolverState *st) {
 }
 
 // Progress for the host's throttling (it keeps a bounded number of
-// iterations queued ahead of the device): a mapped word, so no event has to
-// be recorded in the stream.
+// half-iterations queued ahead of the device): a mapped word, so no event
+// has to be recorded in the stream.
 __device__ inline void signal_progress(SolverState *st, int32_t count) {
     if (st->host_prog)
         __hip_atomic_store(st->host_prog, ((uint64_t)(uint32_t)st->run_id << 32) | (uint32_t)count, __ATOMIC_RELAXED,
@@ -156,7 +156,10 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
         if (!finite(nrm)) { st->nan_flag = 1; signal_done(st); }
         else if (!(nrm > st->thr)) { st->converged = 1; signal_done(st); }
         else if ((st->h >> 1) == st->maxit) { st->max_iter = 1; signal_done(st); }
-        else st->h += 1;
+        else {
+            st->h += 1;
+            signal_progress(st, st->h);
+        }
         break;
     }
     case FIN_OMEGA:    // step 10

@@ -523,11 +523,29 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     }
     // Iterations are queued ahead of the device; the device stops itself
     // (every kernel returns at once after the stop) and raises a mapped host
-    // flag.  The host keeps at most `ahead` iterations queued beyond the one
-    // running (bda_set_options' poll_iters, 1..3) and checks the flag each
-    // time it waits.
+    // flag.  The host keeps at most 2 * ahead - 1 half-iterations queued
+    // beyond the one running (ahead = bda_set_options' poll_iters, 1..3): it
+    // waits on the mapped progress word (no stream event), which the scalar
+    // step of every half-iteration advances, and checks the stop flag each
+    // time.  What is still queued when the device stops runs as no-op
+    // launches (a few microseconds each), so the queue is kept short.
     bool stop = false;
     const int ahead = std::min(3, std::max(1, h.poll_iters));
+    const int ahead_h = 2 * ahead - 1;
+    auto throttle = [&](int half) {
+        if (half > ahead_h && !wait_progress(h, 0, run_id, half - ahead_h, t0)) {
+            out.aborted = true;
+            stop = true;
+        }
+        if (run_stopped(h, 0, run_id)) stop = true;
+        if (h.timeout_s > 0.0) {
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (el > h.timeout_s) {
+                out.aborted = true;
+                stop = true;
+            }
+        }
+    };
     for (int it = 1; it <= maxit + 1 && !stop; ++it) {
         h.prof_tag = 2 * (it - 1);
         precond<BS, TV, TX>(h, op, k.p.p, k.y.p, k.phat.p, true);
@@ -539,6 +557,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         check_launch();
         launch_finalize(h, 1, red(h, 0, FIN_HALF1), vb);
         prof_end(h);
+        throttle(2 * it - 1);
+        if (stop) break;
         h.prof_tag = 2 * (it - 1) + 1;
         precond<BS, TV, TX>(h, op, k.s.p, k.y.p, k.shat.p, true);
         launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p,
@@ -552,21 +572,7 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                            k.v.p, (const SolverState *)(h.st.p));
         check_launch();
         prof_end(h);
-        // keep at most `ahead` iterations queued beyond the one running:
-        // wait (on the mapped progress word, no stream event) until the
-        // device has finished iteration it - ahead
-        if (it > ahead && !wait_progress(h, 0, run_id, 2 * (it - ahead), t0)) {
-            out.aborted = true;
-            stop = true;
-        }
-        if (run_stopped(h, 0, run_id)) stop = true;
-        if (h.timeout_s > 0.0) {
-            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (el > h.timeout_s) {
-                out.aborted = true;
-                stop = true;
-            }
-        }
+        throttle(2 * it);
     }
     if (outer) {
         h.prof_tag = -1;
