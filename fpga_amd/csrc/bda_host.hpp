@@ -19,6 +19,9 @@ constexpr int kTileRows = 64;
 // A third, tall tiling (narrow block capacity, up to 128 rows) serves sweep
 // colours with few blocks per row, whose narrow tiles are short of work.
 constexpr int kTileRowsTall = 128;
+// The merged launch around the last colour handles at most this many rows
+// with blocks in colour C-2 of a second sweep (one per thread group of BS).
+constexpr int kTailRows = 64;
 inline int tile_cap(int bs, bool wide = false) { return (bs == 3 ? 168 : 128) * (wide ? 2 : 1); }
 
 struct Bcsr {

@@ -461,8 +461,12 @@ template <int BS, int MODE, typename TV, typename TX> struct StreamWaves {
 //
 // ROWS = 128 (the tall tiling, used by sweep colours with few blocks per row)
 // gives every thread two rows of the epilogue, 64 rows apart.
-template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
-__global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamWaves<BS, MODE, TV, TX>::value))) void k_stream(StreamArgs<TV, TX> a) {
+//
+// stream_tile is the work of one workgroup on the tile with descriptor d;
+// with KEEP it also leaves the tile's outputs (rounded to TX, (row - first
+// row) * BS + component) in keep[], an LDS array of 64 * BS entries.
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS, bool KEEP>
+__device__ __attribute__((always_inline)) inline void stream_tile(const StreamArgs<TV, TX> &a, const int4 d, TX *keep) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
@@ -486,13 +490,7 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     __shared__ int32_t srow[NT];
     __shared__ double tv[MODE == SM_SUB_DINV ? RS * NT : 1];
 
-    // the stop flag and the tile descriptor are loaded together (one scalar
-    // round trip instead of two before the first vector load): the exit test
-    // also reads the descriptor (a negative entry never occurs), so the
-    // compiler cannot sink its load below the branch
-    const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
-    const int4 d = a.tile[tile];
-    if ((a.ra.st->done | ((d.x | d.y | d.z | d.w) < 0 ? 1 : 0)) != 0) return;
+    static_assert(!KEEP || ROWS == 64, "kept outputs: one row per thread");
     const int r0 = d.x, r1 = d.z, b0 = d.y;
     const int nblk = d.w - b0;
     const int nrow = r1 - r0;
@@ -619,6 +617,7 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
         if (!has_row[s]) continue;
         const TX o = from_double<TX>(res[s]);
         a.out[gi[s]] = o;
+        if constexpr (KEEP) keep[t] = o;
         if (a.out2) a.out2[gi[s]] = o;
         if (a.out3) a.out3[gi[s]] = o;
         const double od = (double)o;
@@ -637,6 +636,107 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
         block_partials<NT, 1>(v1, a.ra);
     } else if constexpr (NRED == 2) {
         block_partials<NT, 2>(red, a.ra);
+    }
+}
+
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
+__global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamWaves<BS, MODE, TV, TX>::value))) void k_stream(StreamArgs<TV, TX> a) {
+    // the stop flag and the tile descriptor are loaded together (one scalar
+    // round trip instead of two before the first vector load): the exit test
+    // also reads the descriptor (a negative entry never occurs), so the
+    // compiler cannot sink its load below the branch
+    const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
+    const int4 d = a.tile[tile];
+    if ((a.ra.st->done | ((d.x | d.y | d.z | d.w) < 0 ? 1 : 0)) != 0) return;
+    stream_tile<BS, MODE, RED, TV, TX, ROWS, false>(a, d, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// The two sweep launches around the last colour C-1, merged.  When the
+// first sweep's last colour is one tile and at most 64 rows of colour C-2
+// carry blocks of the second sweep (their blocks can only point into colour
+// C-1), one launch does:
+//  - workgroup 0: the first sweep's tile of colour C-1 (which writes z
+//    there), then, from those z values kept in LDS, the second sweep's rows
+//    of colour C-2 that carry blocks;
+//  - the other workgroups: the second sweep's rows of colour C-2 without
+//    blocks, z_i = D_i^-1 y_i (or y_i without a diagonal inverse).
+// Every row is computed with the arithmetic of k_stream, in the same order.
+template <typename TV, typename TX>
+struct TailArgs {
+    const int32_t *rowptr;   // second-sweep pattern
+    const int32_t *colidx;
+    const TV *val;
+    const TX *rhs;           // second sweep: out = [D^-1] (rhs - S2 out)
+    TX *out;
+    const double *dinv;
+    int32_t row0, nrows;     // the rows of colour C-2
+    const int32_t *dep;      // its rows that carry blocks (ndep <= 64)
+    int32_t ndep;
+};
+
+template <int BS, int MODEA, int MODEB, typename TV, typename TX>
+__global__ __launch_bounds__(64 * BS) void k_tail(StreamArgs<TV, TX> a, TailArgs<TV, TX> b) {
+    constexpr int BB = BS * BS;
+    constexpr int NT = 64 * BS;
+    __shared__ TX zk[64 * BS];
+    __shared__ double tvb[NT];
+    if (a.ra.st->done) return;
+    const int t = threadIdx.x;
+    if (blockIdx.x == 0) {
+        const int4 d = a.tile[a.tile0];
+        stream_tile<BS, MODEA, SR_NONE, TV, TX, 64, true>(a, d, zk);
+        __syncthreads();
+        const int r0 = d.x;
+        const int k = t / BS, r = t - k * BS;
+        const bool has = k < b.ndep;
+        const int i = b.ndep > 0 ? b.dep[min(k, b.ndep - 1)] : b.row0;
+        const size_t gi = (size_t)i * BS + r;
+        double res = 0.0;
+        if (has) {
+            double sum = 0.0;
+            for (int q = b.rowptr[i]; q < b.rowptr[i + 1]; ++q) {
+                const int j = b.colidx[q] - r0;
+                const TV *av = b.val + (size_t)q * BB + r * BS;
+                double sm = prod<TV, TX>(av[0], zk[j * BS]);
+                sm = sm + prod<TV, TX>(av[1], zk[j * BS + 1]);
+                sm = sm + prod<TV, TX>(av[2], zk[j * BS + 2]);
+                if constexpr (BS == 4) sm = sm + prod<TV, TX>(av[3], zk[j * BS + 3]);
+                sum += sm;
+            }
+            res = (double)b.rhs[gi] - sum;
+        }
+        if constexpr (MODEB == SM_SUB_DINV) {
+            tvb[t] = res;
+            __syncthreads();
+            if (has) {
+                const double *D = b.dinv + gi * BS;
+                double z = 0.0;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) z += D[c] * tvb[k * BS + c];
+                res = z;
+            }
+        }
+        if (has) b.out[gi] = from_double<TX>(res);
+    } else {
+        const int i = b.row0 + (int)(blockIdx.x - 1) * NT + t;
+        if (i < b.row0 + b.nrows && b.rowptr[i + 1] == b.rowptr[i]) {
+            double res[BS];
+#pragma unroll
+            for (int c = 0; c < BS; ++c) res[c] = (double)b.rhs[(size_t)i * BS + c] - 0.0;
+#pragma unroll
+            for (int rr = 0; rr < BS; ++rr) {
+                double o = res[rr];
+                if constexpr (MODEB == SM_SUB_DINV) {
+                    const double *D = b.dinv + ((size_t)i * BS + rr) * BS;
+                    double z = 0.0;
+#pragma unroll
+                    for (int c = 0; c < BS; ++c) z += D[c] * res[c];
+                    o = z;
+                }
+                b.out[(size_t)i * BS + rr] = from_double<TX>(o);
+            }
+        }
     }
 }
 

@@ -67,6 +67,11 @@ struct DevMat {
     DBuf<int4> tile_tall;                // the tall tiling (sweeps, colour order only)
     std::vector<int32_t> colour_tile, colour_tile_wide, colour_tile_tall;
     int ntiles_wide = 0;
+    // rows of colour C-2 that carry blocks (for the merged tail launch of a
+    // second sweep), when there are at most kTailRows of them; else -1
+    DBuf<int32_t> tail_dep;
+    int tail_ndep = -1;
+    int tail_row0 = 0, tail_nrows = 0;
 
     // Spatial order: tiles sorted by the original index of their first row.
     // A launch over all rows has no ordering constraint, and in this order
@@ -118,6 +123,28 @@ struct DevMat {
         upload_tiles(p, p.tile_row_wide, orig_row, tile_wide, &tile_wide_sp, s);
         upload_tiles(p, p.tile_row_tall, nullptr, tile_tall, nullptr, s);
     }
+    void set_tail(const MatPattern &p, const std::vector<int32_t> &colour_ptr, hipStream_t s) {
+        tail_ndep = -1;
+        const int nc = (int)colour_ptr.size() - 1;
+        if (nc < 3) return;
+        tail_row0 = colour_ptr[nc - 2];
+        tail_nrows = colour_ptr[nc - 1] - tail_row0;
+        const int last0 = colour_ptr[nc - 1], last1 = colour_ptr[nc];
+        if (last1 - last0 > kTileRows) return;
+        std::vector<int32_t> dep;
+        for (int r = tail_row0; r < tail_row0 + tail_nrows; ++r) {
+            if (p.rowptr[r + 1] == p.rowptr[r]) continue;
+            for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q)
+                if (p.colidx[q] < last0 || p.colidx[q] >= last1) return;  // blocks only into colour C-1
+            dep.push_back(r);
+        }
+        if ((int)dep.size() > kTailRows) return;
+        tail_dep.ensure(std::max<size_t>(dep.size(), 1));
+        if (!dep.empty())
+            HIP_CHECK(hipMemcpyAsync(tail_dep.p, dep.data(), dep.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        tail_ndep = (int)dep.size();
+    }
     void release() {
         rowptr.release();
         colidx.release();
@@ -126,6 +153,7 @@ struct DevMat {
         tile_wide.release();
         tile_wide_sp.release();
         tile_tall.release();
+        tail_dep.release();
         src.release();
     }
 };
@@ -402,6 +430,58 @@ void sweep_launch(H &h, const DevMat &m, const TV *val, int c, bool dinv_mode, c
                                                    nullptr, x_lo, split);
 }
 
+// The first sweep's last colour C-1 and the second sweep's colour C-2 in one
+// launch (k_tail): workgroup 0 runs the single tile of colour C-1 and then
+// the rows of colour C-2 that carry second-sweep blocks; the other
+// workgroups, one row per thread, the rows of colour C-2 without blocks.
+template <int BS, int MODEA, int MODEB, typename TV, typename TX>
+void tail_dispatch(H &h, const StreamArgs<TV, TX> &a, const TailArgs<TV, TX> &b) {
+    constexpr int NT = 64 * BS;
+    const int grid = 1 + (b.nrows + NT - 1) / NT;
+    hipLaunchKernelGGL((k_tail<BS, MODEA, MODEB, TV, TX>), dim3(grid), dim3(NT), 0, h.stream, a, b);
+    check_launch();
+}
+
+template <int BS, typename TV, typename TX>
+void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split, const TX *p, TX *z,
+                 bool use_state) {
+    const int nc = h.an.ncolours;
+    const DevMat &m1 = *op.S1, &m2 = *op.S2;
+    StreamArgs<TV, TX> a;
+    std::memset(&a, 0, sizeof(a));
+    a.tile = m1.tile.p;
+    a.tile0 = m1.colour_tile[nc - 1];
+    a.ntiles = 1;
+    a.rowptr = m1.rowptr.p;
+    a.colidx = m1.colidx.p;
+    a.val = op.S1val;
+    a.x = y;
+    a.x_lo = lo ? lo : y;
+    a.split = lo ? split : 0;
+    a.rhs = p;
+    a.out = z;
+    a.dinv = op.dinv;
+    a.ra = red(h, 0, FIN_NONE);
+    if (!use_state) a.ra.st = h.st.p + kIdleState;
+    TailArgs<TV, TX> b;
+    b.rowptr = m2.rowptr.p;
+    b.colidx = m2.colidx.p;
+    b.val = op.S2val;
+    b.rhs = y;
+    b.out = z;
+    b.dinv = op.dinv;
+    b.row0 = m2.tail_row0;
+    b.nrows = m2.tail_nrows;
+    b.dep = m2.tail_dep.p;
+    b.ndep = m2.tail_ndep;
+    if (b.ndep < 0 || b.ndep > kTailRows || m1.colour_tile[nc] - m1.colour_tile[nc - 1] != 1 ||
+        b.row0 != h.an.colour_ptr[nc - 2] || b.row0 + b.nrows != h.an.colour_ptr[nc - 1])
+        throw std::runtime_error("merged tail launch does not fit the colouring");
+    if (op.s2_dinv) tail_dispatch<BS, SM_SUB_DINV, SM_SUB_DINV, TV, TX>(h, a, b);
+    else if (op.s1_dinv) tail_dispatch<BS, SM_SUB_DINV, SM_SUB, TV, TX>(h, a, b);
+    else tail_dispatch<BS, SM_SUB, SM_SUB, TV, TX>(h, a, b);
+}
+
 // z = M^{-1} p through the two colour sweeps of op (first sweep colours
 // ascending into y, second sweep colours descending into z).  Two launches
 // are saved per application:
@@ -411,6 +491,9 @@ void sweep_launch(H &h, const DevMat &m, const TV *val, int c, bool dinv_mode, c
 //  - the last colour has no blocks in the second sweep (nothing follows
 //    it), so its first-sweep launch writes z directly, applying the second
 //    sweep's diagonal inverse in the same epilogue when it has one.
+// When the last colour is a single tile and few rows of colour C-2 have
+// second-sweep blocks, that launch and the second sweep's colour C-2 launch
+// are one (tail_launch).
 template <int BS, typename TV, typename TX>
 void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_state) {
     const int nc = h.an.ncolours;
@@ -418,14 +501,18 @@ void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_stat
     const bool skip0 = !op.s1_dinv && nc > 1;
     const TX *lo = skip0 ? p : nullptr;
     const int split = skip0 ? h.an.colour_ptr[1] : 0;
+    const bool merge = nc >= 3 && op.S2->tail_ndep >= 0 &&
+                       op.S1->colour_tile[nc] - op.S1->colour_tile[nc - 1] == 1;
     prof_begin(h, PK_LOWER);
     for (int c = skip0 ? 1 : 0; c < nc - 1; ++c)
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
-    sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
-                             use_state);
+    if (!merge)
+        sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
+                                 use_state);
     prof_end(h);
     prof_begin(h, PK_UPPER);
-    for (int c = nc - 2; c >= 0; --c)
+    if (merge) tail_launch<BS, TV, TX>(h, op, y, lo, split, p, z, use_state);
+    for (int c = merge ? nc - 3 : nc - 2; c >= 0; --c)
         sweep_launch<BS, TV, TX>(h, *op.S2, op.S2val, c, op.s2_dinv, z, nullptr, 0, (c == 0 && skip0) ? p : y, z,
                                  op.dinv, use_state);
     prof_end(h);
@@ -640,6 +727,8 @@ void do_analyse(H &h, bool level) {
     h.At.upload(an.At, h.stream, &an.perm);
     h.Ut.upload(an.Ut, h.stream);
     h.Lt.upload(an.Lt, h.stream);
+    h.U.set_tail(an.U, an.colour_ptr, h.stream);
+    h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
     h.d_diag.ensure(h.n);
     HIP_CHECK(hipMemcpy(h.d_perm.p, an.perm.data(), h.n * 4, hipMemcpyHostToDevice));
