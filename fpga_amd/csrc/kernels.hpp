@@ -8,7 +8,9 @@
 //   ILU0_L_fs, ilu0.vhd:412-416                               -> k_stream<SUB> per colour
 //   ILU0_U_bs, ilu0.vhd:385-410                               -> k_stream<SUB_DINV> per colour
 //   dot1 -> alpha, solver.vhd:793-798                         -> SpMV epilogue + FIN_ALPHA
-//   axpy1 + axpy2/norm, solver_pkg.vhd:128-130                -> k_update_xs + FIN_HALF1
+//   axpy1 + axpy2/norm, solver_pkg.vhd:128-130                -> k_update_s + FIN_HALF1 (axpy1,
+//                                                                 x += alpha phat, deferred into
+//                                                                 k_update_xr / k_finish_x)
 //   dot2 -> omega, solver.vhd:831-842                         -> SpMV epilogue + FIN_OMEGA
 //   axpy1 + axpy2/norm + dot1 -> beta, solver_pkg.vhd:133-136 -> k_update_xr + FIN_HALF2
 //   calc_p, vector_ops.vhd:319-323,429-458                    -> k_update_p
@@ -791,19 +793,18 @@ __device__ inline VecIdx vec_index(int64_t N) {
     return v;
 }
 
-// steps 6-8: x += alpha phat; s = r - alpha v; ||s||^2
-template <typename TX, typename TXX>
-__global__ __launch_bounds__(kVecThreads) void k_update_xs(int64_t N, TXX *x, const TX *phat, const TX *r,
-                                                          const TX *v, TX *s, RedArgs ra) {
+// steps 6-8 without the x update: s = r - alpha v; ||s||^2.  Step 6,
+// x += alpha phat, is deferred: k_update_xr applies it together with step 11
+// (the same two roundings, one read and one write of x fewer), and
+// k_finish_x applies it alone when a run ends after a first half-iteration.
+template <typename TX>
+__global__ __launch_bounds__(kVecThreads) void k_update_s(int64_t N, const TX *r, const TX *v, TX *s, RedArgs ra) {
     if (ra.st->done) return;
-    const double alpha = ra.st->alpha, malpha = -alpha;
+    const double malpha = -ra.st->alpha;
     const VecIdx ix = vec_index(N);
-    double x0[kVecPairsPerThread], x1[kVecPairsPerThread], p0[kVecPairsPerThread], p1[kVecPairsPerThread];
     double r0[kVecPairsPerThread], r1[kVecPairsPerThread], v0[kVecPairsPerThread], v1[kVecPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
-        ld2(x, ix.pi[j], x0[j], x1[j]);
-        ld2(phat, ix.pi[j], p0[j], p1[j]);
         ld2(r, ix.pi[j], r0[j], r1[j]);
         ld2(v, ix.pi[j], v0[j], v1[j]);
     }
@@ -811,7 +812,6 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xs(int64_t N, TXX *x, co
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
         const double s0 = rnd<TX>(malpha * v0[j] + r0[j]), s1 = rnd<TX>(malpha * v1[j] + r1[j]);
-        st2(x, ix.pi[j], alpha * p0[j] + x0[j], alpha * p1[j] + x1[j]);
         st2(s, ix.pi[j], s0, s1);
         acc += ix.ok[j] ? s0 * s0 : 0.0;
         acc += ix.second[j] ? s1 * s1 : 0.0;
@@ -820,19 +820,22 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xs(int64_t N, TXX *x, co
     block_partials<kVecThreads, 1>(vv, ra);
 }
 
-// steps 11-14: x += omega shat; r = s - omega t; ||r||^2 and rt.r
+// steps 6 and 11-14: x = (x + alpha phat) + omega shat, rounded to the type
+// of x after each addition as two separate updates would be; r = s - omega
+// t; ||r||^2 and rt.r
 template <typename TX, typename TXX>
-__global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, const TX *shat, const TX *s,
-                                                          const TX *t, TX *r, const TX *rt, RedArgs ra) {
+__global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, const TX *phat, const TX *shat,
+                                                          const TX *s, const TX *t, TX *r, const TX *rt, RedArgs ra) {
     if (ra.st->done) return;
-    const double omega = ra.st->omega, momega = -omega;
+    const double alpha = ra.st->alpha, omega = ra.st->omega, momega = -omega;
     const VecIdx ix = vec_index(N);
     double x0[kVecPairsPerThread], x1[kVecPairsPerThread], h0[kVecPairsPerThread], h1[kVecPairsPerThread];
     double s0[kVecPairsPerThread], s1[kVecPairsPerThread], t0[kVecPairsPerThread], t1[kVecPairsPerThread];
-    double q0[kVecPairsPerThread], q1[kVecPairsPerThread];
+    double q0[kVecPairsPerThread], q1[kVecPairsPerThread], p0[kVecPairsPerThread], p1[kVecPairsPerThread];
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
         ld2(x, ix.pi[j], x0[j], x1[j]);
+        ld2(phat, ix.pi[j], p0[j], p1[j]);
         ld2(shat, ix.pi[j], h0[j], h1[j]);
         ld2(s, ix.pi[j], s0[j], s1[j]);
         ld2(t, ix.pi[j], t0[j], t1[j]);
@@ -842,7 +845,8 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, co
 #pragma unroll
     for (int j = 0; j < kVecPairsPerThread; ++j) {
         const double ra0 = rnd<TX>(momega * t0[j] + s0[j]), ra1 = rnd<TX>(momega * t1[j] + s1[j]);
-        st2(x, ix.pi[j], omega * h0[j] + x0[j], omega * h1[j] + x1[j]);
+        const double xa0 = rnd<TXX>(alpha * p0[j] + x0[j]), xa1 = rnd<TXX>(alpha * p1[j] + x1[j]);
+        st2(x, ix.pi[j], omega * h0[j] + xa0, omega * h1[j] + xa1);
         st2(r, ix.pi[j], ra0, ra1);
         acc[0] += ix.ok[j] ? ra0 * ra0 : 0.0;
         acc[1] += ix.ok[j] ? q0[j] * ra0 : 0.0;
@@ -850,6 +854,24 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, co
         acc[1] += ix.second[j] ? q1[j] * ra1 : 0.0;
     }
     block_partials<kVecThreads, 2>(acc, ra);
+}
+
+// End of a run: the deferred step 6, x += alpha phat, when the last
+// half-iteration that ran was a first half (its ||s|| test stopped the run,
+// or the run ended there); it runs whether or not the run has stopped.
+template <typename TX, typename TXX>
+__global__ __launch_bounds__(kVecThreads) void k_finish_x(int64_t N, TXX *x, const TX *phat, const SolverState *st) {
+    if (st->last_half != 1) return;
+    const double alpha = st->alpha;
+    const VecIdx ix = vec_index(N);
+    double x0[kVecPairsPerThread], x1[kVecPairsPerThread], p0[kVecPairsPerThread], p1[kVecPairsPerThread];
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        ld2(x, ix.pi[j], x0[j], x1[j]);
+        ld2(phat, ix.pi[j], p0[j], p1[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) st2(x, ix.pi[j], alpha * p0[j] + x0[j], alpha * p1[j] + x1[j]);
 }
 
 // steps 15-17: p = r + beta (p - omega v), as tmp = (-omega) v + p; p = beta tmp + r

@@ -639,8 +639,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p,
                                                   red(h, 0, FIN_ALPHA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
-        hipLaunchKernelGGL((k_update_xs<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
-                           k.phat.p, k.r.p, k.v.p, k.s.p, red(h, 0, FIN_HALF1));
+        hipLaunchKernelGGL((k_update_s<TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.r.p, k.v.p,
+                           k.s.p, red(h, 0, FIN_HALF1));
         check_launch();
         launch_finalize(h, 1, red(h, 0, FIN_HALF1), vb);
         prof_end(h);
@@ -652,7 +652,7 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                                                     red(h, 0, FIN_OMEGA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
         hipLaunchKernelGGL((k_update_xr<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p,
-                           k.shat.p, k.s.p, k.t.p, k.r.p, k.rt.p, red(h, 0, FIN_HALF2));
+                           k.phat.p, k.shat.p, k.s.p, k.t.p, k.r.p, k.rt.p, red(h, 0, FIN_HALF2));
         check_launch();
         launch_finalize(h, 2, red(h, 0, FIN_HALF2), vb);
         hipLaunchKernelGGL((k_update_p<TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.p.p, k.r.p,
@@ -661,6 +661,11 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         prof_end(h);
         throttle(2 * it);
     }
+    // x += alpha phat of a run that ended after a first half-iteration (the
+    // x update of a first half is otherwise applied by the next k_update_xr)
+    hipLaunchKernelGGL((k_finish_x<TX, TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N, k.x.p, k.phat.p,
+                       (const SolverState *)(h.st.p));
+    check_launch();
     if (outer) {
         h.prof_tag = -1;
         return out;
