@@ -96,3 +96,59 @@ def test_solve_parity(pair):
     xm, rm = g.solve_mixed(b, tol=1e-8)
     assert rm.converged
     np.testing.assert_allclose(xm, xo, rtol=0, atol=1e-6 * np.abs(xo).max())
+
+
+def fan_coupled(bs, nlast, fan, n=2000, seed=1):
+    """Rows 1000 + q (q < 200) couple to q and take colour 1; rows 1500 + i
+    (i < nlast) couple to i and to the `fan` colour-1 rows 1000 + i*fan + j,
+    and take colour 2.  Colour 1 then has nlast * fan rows with blocks into
+    the last colour: 64 of them still fit the merged launch of the last
+    colour with colour 1's upper sweep, 70 take the separate launches."""
+    rng = np.random.default_rng(seed)
+    nb = [set([i]) for i in range(n)]
+    for q in range(200):
+        nb[1000 + q].add(q); nb[q].add(1000 + q)
+    for i in range(nlast):
+        nb[1500 + i].add(i); nb[i].add(1500 + i)
+        for j in range(fan):
+            c = 1000 + i * fan + j
+            nb[1500 + i].add(c); nb[c].add(1500 + i)
+    rowptr = np.zeros(n + 1, np.int32)
+    cols = []
+    for i in range(n):
+        c = sorted(nb[i])
+        cols.extend(c)
+        rowptr[i + 1] = rowptr[i] + len(c)
+    colidx = np.array(cols, np.int32)
+    vals = rng.uniform(-0.5, 0.5, (len(cols), bs, bs))
+    for i in range(n):
+        qd = [q for q in range(rowptr[i], rowptr[i + 1]) if colidx[q] == i][0]
+        off = sum((np.abs(vals[q]).sum(axis=1) for q in range(rowptr[i], rowptr[i + 1]) if q != qd), np.zeros(bs))
+        vals[qd] = np.diag(off + 1.0) + rng.normal(0.0, 0.05, (bs, bs))
+    return rowptr, colidx, vals
+
+
+@pytest.mark.parametrize("bs", [3, 4])
+@pytest.mark.parametrize("nlast,fan", [(8, 8), (10, 7)])
+def test_precond_bit_exact_tail_limit(bs, nlast, fan):
+    print(f"case fan_coupled bs={bs} nlast={nlast} fan={fan}", flush=True)
+    rowptr, colidx, vals = fan_coupled(bs, nlast, fan)
+    g = fa.BicgstabSolver(bs, len(rowptr) - 1, len(colidx))
+    try:
+        g.upload_bcsr(rowptr, colidx, vals)
+        g.analyse()
+        g.factor()
+        _, _, cptr = g.colouring()
+        np.testing.assert_array_equal(np.diff(cptr), [2000 - 200 - nlast, 200, nlast])
+        o = oracle.OracleSystem(bs, rowptr, colidx, vals)
+        o.factor()
+        p = np.random.default_rng(7).uniform(-1, 1, (len(rowptr) - 1) * bs)
+        np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
+        np.testing.assert_array_equal(g.precond_apply_perm(p, transpose=True), o.precond_perm(p, transpose=True))
+        b = np.random.default_rng(8).uniform(-1, 1, len(p))
+        xg, rg = g.solve(b, tol=1e-10, maxit=200)
+        xo, ro = o.solve(b, tol=1e-10, maxit=200)
+        assert rg.converged and ro["converged"]
+        np.testing.assert_allclose(xg, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
+    finally:
+        g.close()
