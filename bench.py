@@ -207,10 +207,21 @@ def main():
             t0 = time.perf_counter()
             xo, ro = o.solve(s.b, tol=tol, maxit=maxit)
             times.append(time.perf_counter() - t0)
+        # the same oracle on one thread, over a bounded sample: 3 fixed iterations
+        oracle.set_threads(1)
+        t0 = time.perf_counter()
+        o.solve(s.b, tol=0.0, maxit=3)
+        per_it_1 = (time.perf_counter() - t0) / 3
+        oracle.set_threads(threads)
         cpu = {"value": min(times), "unit": "s", "cores": threads, "kind": "port",
                "sample": f"full FP64 solve of {args.workload} to rel 1e-6 ({ro['iterations']} iterations), "
-                         f"best of 2, OpenMP over rows"}
-        log(f"cpu oracle: {times} s, iterations {ro['iterations']}")
+                         f"best of 2, OpenMP over rows",
+               "s_per_iteration": min(times) / ro["iterations"],
+               "nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+               "one_thread": {"s_per_iteration": per_it_1,
+                              "solve_s_estimate": per_it_1 * ro["iterations"],
+                              "sample": "3 fixed iterations (tol 0) on 1 thread, setup included"}}
+        log(f"cpu oracle: {times} s, iterations {ro['iterations']}; 1 thread {per_it_1:.3f} s per iteration")
 
     out = {
         "metric": "time-to-1e-6 rel-res (s), mixed vs FP64 on GPU; achieved HBM GB/s",
