@@ -268,7 +268,7 @@ __device__ inline double wave_sum(double v) {
 // partials are summed in a fixed order, so results do not depend on
 // workgroup scheduling.
 template <int NT, int NRED>
-__device__ inline void block_partials(double (&v)[NRED], const RedArgs &ra) {
+__device__ inline void block_partials(double (&v)[NRED], const RedArgs &ra, int slot) {
     __shared__ double sred[NT / 64][2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -282,7 +282,7 @@ __device__ inline void block_partials(double (&v)[NRED], const RedArgs &ra) {
         for (int j = 0; j < NRED; ++j) {
             double t = sred[0][j];
             for (int w = 1; w < NT / 64; ++w) t += sred[w][j];
-            ra.partials[(size_t)j * ra.cap + blockIdx.x] = t;
+            ra.partials[(size_t)j * ra.cap + slot] = t;
         }
     }
 }
@@ -351,6 +351,7 @@ enum StreamMode : int { SM_SPMV = 0, SM_SUB = 1, SM_SUB_DINV = 2 };
 template <typename TV, typename TX>
 struct StreamArgs {
     const int4 *tile;   // tile descriptors {first row, first block, end row, end block}
+    const int32_t *slot;// reduction slot of each tile: its place in the colour-ordered tile list
     int32_t tile0, ntiles;
     const int32_t *rowptr;
     const int32_t *colidx;
@@ -437,6 +438,7 @@ template <int BS, int MODE> constexpr int stream_cap() { return TileCap<BS, wide
 //
 // Reductions fused into the epilogue: bit 0 = sum out*w0, bit 1 = sum out*out.
 enum StreamRed : int { SR_NONE = 0, SR_W0 = 1, SR_SELF = 2, SR_BOTH = 3 };
+template <int RED> struct NRedOf { static constexpr int value = (RED & 1) + ((RED >> 1) & 1); };
 
 // Minimum waves per SIMD requested from the register allocator: 8 for the
 // FP32 3x3 product kernel, whose natural allocation lands just above 64
@@ -466,9 +468,11 @@ template <int BS, int MODE, typename TV, typename TX> struct StreamWaves {
 //
 // stream_tile is the work of one workgroup on the tile with descriptor d;
 // with KEEP it also leaves the tile's outputs (rounded to TX, (row - first
-// row) * BS + component) in keep[], an LDS array of 64 * BS entries.
+// row) * BS + component) in keep[], an LDS array of 64 * BS entries.  Its
+// fused reductions write partial-sum slot `slot`.
 template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS, bool KEEP>
-__device__ __attribute__((always_inline)) inline void stream_tile(const StreamArgs<TV, TX> &a, const int4 d, TX *keep) {
+__device__ __attribute__((always_inline)) inline void stream_tile(const StreamArgs<TV, TX> &a, const int4 d, TX *keep,
+                                                                   int slot) {
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
@@ -635,9 +639,9 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     }
     if constexpr (NRED == 1) {
         double v1[1] = {red[0]};
-        block_partials<NT, 1>(v1, a.ra);
+        block_partials<NT, 1>(v1, a.ra, slot);
     } else if constexpr (NRED == 2) {
-        block_partials<NT, 2>(red, a.ra);
+        block_partials<NT, 2>(red, a.ra, slot);
     }
 }
 
@@ -650,7 +654,10 @@ __global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamW
     const int tile = a.tile0 + xcd_tile(blockIdx.x, a.ntiles);
     const int4 d = a.tile[tile];
     if ((a.ra.st->done | ((d.x | d.y | d.z | d.w) < 0 ? 1 : 0)) != 0) return;
-    stream_tile<BS, MODE, RED, TV, TX, ROWS, false>(a, d, nullptr);
+    // the partial sums go to the tile's own slot, so that the fixed-order sum
+    // over them does not depend on the order in which tiles are launched
+    const int slot = NRedOf<RED>::value > 0 ? a.slot[tile] : 0;
+    stream_tile<BS, MODE, RED, TV, TX, ROWS, false>(a, d, nullptr, slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -687,7 +694,7 @@ __global__ __launch_bounds__(64 * BS) void k_tail(StreamArgs<TV, TX> a, TailArgs
     const int t = threadIdx.x;
     if (blockIdx.x == 0) {
         const int4 d = a.tile[a.tile0];
-        stream_tile<BS, MODEA, SR_NONE, TV, TX, 64, true>(a, d, zk);
+        stream_tile<BS, MODEA, SR_NONE, TV, TX, 64, true>(a, d, zk, 0);
         __syncthreads();
         const int r0 = d.x;
         const int k = t / BS, r = t - k * BS;
@@ -817,7 +824,7 @@ __global__ __launch_bounds__(kVecThreads) void k_update_s(int64_t N, const TX *r
         acc += ix.second[j] ? s1 * s1 : 0.0;
     }
     double vv[1] = {acc};
-    block_partials<kVecThreads, 1>(vv, ra);
+    block_partials<kVecThreads, 1>(vv, ra, blockIdx.x);
 }
 
 // steps 6 and 11-14: x = (x + alpha phat) + omega shat, rounded to the type
@@ -853,7 +860,7 @@ __global__ __launch_bounds__(kVecThreads) void k_update_xr(int64_t N, TXX *x, co
         acc[0] += ix.second[j] ? ra1 * ra1 : 0.0;
         acc[1] += ix.second[j] ? q1[j] * ra1 : 0.0;
     }
-    block_partials<kVecThreads, 2>(acc, ra);
+    block_partials<kVecThreads, 2>(acc, ra, blockIdx.x);
 }
 
 // End of a run: the deferred step 6, x += alpha phat, when the last
@@ -918,7 +925,7 @@ __global__ __launch_bounds__(kVecThreads) void k_init_zero_x0(int64_t N, const d
         acc += ix.second[j] ? c1 * c1 : 0.0;
     }
     double vv[1] = {acc};
-    block_partials<kVecThreads, 1>(vv, ra);
+    block_partials<kVecThreads, 1>(vv, ra, blockIdx.x);
 }
 
 // x64 += d (outer refinement update of the precision form)

@@ -59,12 +59,16 @@ struct DevMat {
     int64_t nnzb = 0;
     int ntiles = 0;
     DBuf<int32_t> rowptr, colidx, src;
-    // tile descriptors {first row, first block, end row, end block}: in
-    // colour order (per-colour launches address a contiguous range), and the
-    // same tiles in spatial order for launches over the whole matrix
+    // tile descriptors {first row, first block, end row, end block}: grouped
+    // by colour (per-colour launches address a contiguous range), and all
+    // tiles in one list for launches over the whole matrix; both in the
+    // launch order of upload_tiles
     DBuf<int4> tile, tile_sp;
     DBuf<int4> tile_wide, tile_wide_sp;  // the wide tiling
-    DBuf<int4> tile_tall;                // the tall tiling (sweeps, colour order only)
+    DBuf<int4> tile_tall;                // the tall tiling (sweeps: per-colour list only)
+    // reduction slot of every entry of the lists above: the tile's place in
+    // the colour-ordered list (within its colour for the per-colour lists)
+    DBuf<int32_t> slot, slot_sp, slot_wide, slot_wide_sp, slot_tall;
     std::vector<int32_t> colour_tile, colour_tile_wide, colour_tile_tall;
     int ntiles_wide = 0;
     // rows of colour C-2 that carry blocks (for the merged tail launch of a
@@ -73,31 +77,88 @@ struct DevMat {
     int tail_ndep = -1;
     int tail_row0 = 0, tail_nrows = 0;
 
-    // Spatial order: tiles sorted by the original index of their first row.
-    // A launch over all rows has no ordering constraint, and in this order
-    // the tiles that run at the same time (on one XCD) cover neighbouring
-    // rows of every colour, whose x entries then share that XCD's L2
-    // instead of being fetched again for each colour.
+    // Typical coupling distance in the original numbering: the median over
+    // rows of the row's widest coupling (for a structured grid in natural
+    // order, the distance to the neighbour in the next layer).
+    static int64_t coupling_span(const MatPattern &p, const std::vector<int32_t> &o) {
+        std::vector<int32_t> w;
+        w.reserve(p.n);
+        for (int k = 0; k < p.n; ++k) {
+            int32_t m = 0;
+            for (int32_t q = p.rowptr[k]; q < p.rowptr[k + 1]; ++q) m = std::max(m, std::abs(o[k] - o[p.colidx[q]]));
+            if (m > 0) w.push_back(m);
+        }
+        if (w.empty()) return 0;
+        std::nth_element(w.begin(), w.begin() + w.size() / 2, w.end());
+        return w[w.size() / 2];
+    }
+
+    // Launch order of the tiles.  Every tile is independent and its
+    // reduction partials go to a fixed slot, so the order has no effect on
+    // results.  Whole-matrix launches take all tiles, per-colour launches
+    // their colour's tiles, sorted by the original index of the first row
+    // within eighths of a layer (the typical coupling distance).  A launch's
+    // XCDs each work through one contiguous eighth of the list, so each XCD
+    // sweeps one slice of every layer, from the first layer to the last, and
+    // the tiles running at one time on all XCDs cover the same few layers of
+    // every colour.  Measured on a 128x128x64 grid against plain original
+    // order (each XCD one slab of whole layers): FP64 SpMV 99.7 -> 94.9 us,
+    // FP32 SpMV 56.4 -> 54.7 us, FP64 iteration 537 -> 514 us, with 1-2%
+    // more fetched bytes (the gain is not from x reuse).
     static void upload_tiles(const MatPattern &p, const std::vector<int32_t> &tile_row,
-                             const std::vector<int32_t> *orig_row, DBuf<int4> &out, DBuf<int4> *out_sp,
-                             hipStream_t s) {
+                             const std::vector<int32_t> &colour_tile, const std::vector<int32_t> *orig_row,
+                             DBuf<int4> &out, DBuf<int32_t> &out_slot, DBuf<int4> *out_sp,
+                             DBuf<int32_t> *out_sp_slot, hipStream_t s) {
         const int nt = (int)tile_row.size() - 1;
         std::vector<int4> d(std::max(nt, 1), make_int4(0, 0, 0, 0));
         for (int t = 0; t < nt; ++t)
             d[t] = make_int4(tile_row[t], p.rowptr[tile_row[t]], tile_row[t + 1], p.rowptr[tile_row[t + 1]]);
         std::vector<int4> ds = d;
+        std::vector<int32_t> sl(d.size(), 0), sls(d.size(), 0);
+        for (int t = 0; t < nt; ++t) sls[t] = t;
+        for (size_t c = 0; c + 1 < colour_tile.size(); ++c)
+            for (int t = colour_tile[c]; t < colour_tile[c + 1]; ++t) sl[t] = t - colour_tile[c];
         if (orig_row && nt > 0) {
+            const std::vector<int32_t> &o = *orig_row;
+            const int64_t L = coupling_span(p, o);
+            const int64_t n = p.n;
+            // key: slice of the layer (an eighth of L), then original row
+            auto key = [&](int t) -> int64_t {
+                const int64_t r = o[tile_row[t]];
+                const int64_t slice = (L >= 8 && 2 * L <= n) ? ((r % L) * 8) / L : 0;
+                return slice * (n + 1) + r;
+            };
+            auto by_key = [&](int a, int b) { return key(a) < key(b); };
             std::vector<int32_t> ord(nt);
+            // whole-matrix list
             for (int t = 0; t < nt; ++t) ord[t] = t;
-            std::stable_sort(ord.begin(), ord.end(),
-                             [&](int a, int b) { return (*orig_row)[tile_row[a]] < (*orig_row)[tile_row[b]]; });
-            for (int t = 0; t < nt; ++t) ds[t] = d[ord[t]];
+            std::stable_sort(ord.begin(), ord.end(), by_key);
+            for (int t = 0; t < nt; ++t) {
+                ds[t] = d[ord[t]];
+                sls[t] = ord[t];
+            }
+            // per-colour launches: the same order within each colour's range
+            for (int t = 0; t < nt; ++t) ord[t] = t;
+            for (size_t c = 0; c + 1 < colour_tile.size(); ++c)
+                std::stable_sort(ord.begin() + colour_tile[c], ord.begin() + colour_tile[c + 1], by_key);
+            std::vector<int4> dc(d.size());
+            std::vector<int32_t> slc(d.size(), 0);
+            for (int t = 0; t < nt; ++t) {
+                dc[t] = d[ord[t]];
+                slc[t] = sl[ord[t]];
+            }
+            d.swap(dc);
+            sl.swap(slc);
         }
         out.ensure(d.size());
+        out_slot.ensure(sl.size());
         HIP_CHECK(hipMemcpyAsync(out.p, d.data(), d.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(out_slot.p, sl.data(), sl.size() * 4, hipMemcpyHostToDevice, s));
         if (out_sp) {
-            out_sp->ensure(d.size());
+            out_sp->ensure(ds.size());
+            out_sp_slot->ensure(sls.size());
             HIP_CHECK(hipMemcpyAsync(out_sp->p, ds.data(), ds.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+            HIP_CHECK(hipMemcpyAsync(out_sp_slot->p, sls.data(), sls.size() * 4, hipMemcpyHostToDevice, s));
         }
         HIP_CHECK(hipStreamSynchronize(s));
     }
@@ -119,9 +180,10 @@ struct DevMat {
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
-        upload_tiles(p, p.tile_row, orig_row, tile, &tile_sp, s);
-        upload_tiles(p, p.tile_row_wide, orig_row, tile_wide, &tile_wide_sp, s);
-        upload_tiles(p, p.tile_row_tall, nullptr, tile_tall, nullptr, s);
+        upload_tiles(p, p.tile_row, p.colour_tile, orig_row, tile, slot, &tile_sp, &slot_sp, s);
+        upload_tiles(p, p.tile_row_wide, p.colour_tile_wide, orig_row, tile_wide, slot_wide, &tile_wide_sp,
+                     &slot_wide_sp, s);
+        upload_tiles(p, p.tile_row_tall, p.colour_tile_tall, orig_row, tile_tall, slot_tall, nullptr, nullptr, s);
     }
     void set_tail(const MatPattern &p, const std::vector<int32_t> &colour_ptr, hipStream_t s) {
         tail_ndep = -1;
@@ -153,6 +215,7 @@ struct DevMat {
         tile_wide.release();
         tile_wide_sp.release();
         tile_tall.release();
+        for (DBuf<int32_t> *b : {&slot, &slot_sp, &slot_wide, &slot_wide_sp, &slot_tall}) b->release();
         tail_dep.release();
         src.release();
     }
@@ -344,6 +407,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     constexpr bool wide = wide_tiles<MODE>();
     const std::vector<int32_t> &ct = wide ? m.colour_tile_wide : m.colour_tile;
     a.tile = colour < 0 ? (wide ? m.tile_wide_sp.p : m.tile_sp.p) : (wide ? m.tile_wide.p : m.tile.p);
+    a.slot = colour < 0 ? (wide ? m.slot_wide_sp.p : m.slot_sp.p) : (wide ? m.slot_wide.p : m.slot.p);
     if (colour < 0) {
         a.tile0 = 0;
         a.ntiles = wide ? m.ntiles_wide : m.ntiles;
@@ -361,6 +425,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
             if (4 * nt <= 3 * a.ntiles) {
                 tall = true;
                 a.tile = m.tile_tall.p;
+                a.slot = m.slot_tall.p;
                 a.tile0 = m.colour_tile_tall[colour];
                 a.ntiles = nt;
             }
@@ -401,6 +466,7 @@ void launch_stream_timed(H &h, const TV *val, const TX *x, TX *y, const TX *w0, 
     std::memset(&a, 0, sizeof(a));
     constexpr bool wide = wide_tiles<SM_SPMV>();
     a.tile = wide ? h.A.tile_wide_sp.p : h.A.tile_sp.p;
+    a.slot = wide ? h.A.slot_wide_sp.p : h.A.slot_sp.p;
     a.tile0 = 0;
     a.ntiles = wide ? h.A.ntiles_wide : h.A.ntiles;
     a.rowptr = h.A.rowptr.p;
@@ -450,6 +516,7 @@ void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split
     StreamArgs<TV, TX> a;
     std::memset(&a, 0, sizeof(a));
     a.tile = m1.tile.p;
+    a.slot = m1.slot.p;
     a.tile0 = m1.colour_tile[nc - 1];
     a.ntiles = 1;
     a.rowptr = m1.rowptr.p;
@@ -727,11 +794,11 @@ void do_analyse(H &h, bool level) {
     analyse(h.pattern, level, h.an);
     const Analysis &an = h.an;
     h.A.upload(an.A, h.stream, &an.perm);
-    h.L.upload(an.L, h.stream);
-    h.U.upload(an.U, h.stream);
+    h.L.upload(an.L, h.stream, &an.perm);
+    h.U.upload(an.U, h.stream, &an.perm);
     h.At.upload(an.At, h.stream, &an.perm);
-    h.Ut.upload(an.Ut, h.stream);
-    h.Lt.upload(an.Lt, h.stream);
+    h.Ut.upload(an.Ut, h.stream, &an.perm);
+    h.Lt.upload(an.Lt, h.stream, &an.perm);
     h.U.set_tail(an.U, an.colour_ptr, h.stream);
     h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
