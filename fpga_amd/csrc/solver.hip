@@ -111,7 +111,7 @@ struct DevMat {
     // more fetched bytes (the gain is not from x reuse).
     static void upload_tiles(const MatPattern &p, const std::vector<int32_t> &tile_row,
                              const std::vector<int32_t> &colour_tile, const std::vector<int32_t> *orig_row,
-                             DBuf<int4> &out, DBuf<int32_t> &out_slot, DBuf<int4> *out_sp,
+                             int64_t L, DBuf<int4> &out, DBuf<int32_t> &out_slot, DBuf<int4> *out_sp,
                              DBuf<int32_t> *out_sp_slot, hipStream_t s) {
         const int nt = (int)tile_row.size() - 1;
         std::vector<int4> d(std::max(nt, 1), make_int4(0, 0, 0, 0));
@@ -124,7 +124,6 @@ struct DevMat {
             for (int t = colour_tile[c]; t < colour_tile[c + 1]; ++t) sl[t] = t - colour_tile[c];
         if (orig_row && nt > 0) {
             const std::vector<int32_t> &o = *orig_row;
-            const int64_t L = coupling_span(p, o);
             const int64_t n = p.n;
             // key: slice of the layer (an eighth of L), then original row
             auto key = [&](int t) -> int64_t {
@@ -167,8 +166,10 @@ struct DevMat {
         HIP_CHECK(hipStreamSynchronize(s));
     }
 
-    // orig_row: original row of every (permuted) row, for the spatial order
-    void upload(const MatPattern &p, hipStream_t s, const std::vector<int32_t> *orig_row = nullptr) {
+    // orig_row: original row of every (permuted) row, and span: the
+    // matrix's coupling_span, for the launch order of the tiles
+    void upload(const MatPattern &p, hipStream_t s, const std::vector<int32_t> *orig_row = nullptr,
+                int64_t span = 0) {
         n = p.n;
         nnzb = p.nnzb();
         ntiles = p.ntiles();
@@ -184,10 +185,11 @@ struct DevMat {
             HIP_CHECK(hipMemcpyAsync(colidx.p, p.colidx.data(), p.colidx.size() * 4, hipMemcpyHostToDevice, s));
         if (!p.src.empty())
             HIP_CHECK(hipMemcpyAsync(src.p, p.src.data(), p.src.size() * 4, hipMemcpyHostToDevice, s));
-        upload_tiles(p, p.tile_row, p.colour_tile, orig_row, tile, slot, &tile_sp, &slot_sp, s);
-        upload_tiles(p, p.tile_row_wide, p.colour_tile_wide, orig_row, tile_wide, slot_wide, &tile_wide_sp,
+        upload_tiles(p, p.tile_row, p.colour_tile, orig_row, span, tile, slot, &tile_sp, &slot_sp, s);
+        upload_tiles(p, p.tile_row_wide, p.colour_tile_wide, orig_row, span, tile_wide, slot_wide, &tile_wide_sp,
                      &slot_wide_sp, s);
-        upload_tiles(p, p.tile_row_tall, p.colour_tile_tall, orig_row, tile_tall, slot_tall, nullptr, nullptr, s);
+        upload_tiles(p, p.tile_row_tall, p.colour_tile_tall, orig_row, span, tile_tall, slot_tall, nullptr, nullptr,
+                     s);
     }
     void set_tail(const MatPattern &p, const std::vector<int32_t> &colour_ptr, hipStream_t s) {
         tail_ndep = -1;
@@ -797,12 +799,13 @@ void do_analyse(H &h, bool level) {
     if (!h.have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
     analyse(h.pattern, level, h.an);
     const Analysis &an = h.an;
-    h.A.upload(an.A, h.stream, &an.perm);
-    h.L.upload(an.L, h.stream, &an.perm);
-    h.U.upload(an.U, h.stream, &an.perm);
-    h.At.upload(an.At, h.stream, &an.perm);
-    h.Ut.upload(an.Ut, h.stream, &an.perm);
-    h.Lt.upload(an.Lt, h.stream, &an.perm);
+    const int64_t span = DevMat::coupling_span(an.A, an.perm);
+    h.A.upload(an.A, h.stream, &an.perm, span);
+    h.L.upload(an.L, h.stream, &an.perm, span);
+    h.U.upload(an.U, h.stream, &an.perm, span);
+    h.At.upload(an.At, h.stream, &an.perm, span);
+    h.Ut.upload(an.Ut, h.stream, &an.perm, span);
+    h.Lt.upload(an.Lt, h.stream, &an.perm, span);
     h.U.set_tail(an.U, an.colour_ptr, h.stream);
     h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
