@@ -304,7 +304,7 @@ struct bda_handle_s {
     int abs_tol = 0;
     int poll_iters = 1;
     int refine_adaptive = 1;  // last refinement steps ask only what the outer target needs
-    double refine_eta = 0.5;
+    double refine_eta = 0.9;
 
     // profiling
     bool profiling = false;

@@ -271,7 +271,7 @@ class BicgstabSolver:
                     "bda_solve_mixed_dev")
         return SolveResult.from_c(r)
 
-    def set_refinement(self, adaptive: bool = True, eta: float = 0.5):
+    def set_refinement(self, adaptive: bool = True, eta: float = 0.9):
         self._check(lib.bda_set_refinement(self._h, int(adaptive), eta), "bda_set_refinement")
 
     def adjoint_solve(self, w, lam0=None, tol: float = 1e-10, maxit: int = 1000):

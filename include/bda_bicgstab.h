@@ -192,7 +192,10 @@ int bda_solve_mixed_dev(bda_handle h, const double *d_b, double *d_x, double tol
  * inner tolerance of refinement step k is max(inner_tol, eta*thr/||r_k||),
  * capped at 0.5, i.e. a step asks only for the reduction the outer target
  * still needs; adaptive = 0: every inner solve runs to inner_tol.
- * eta in (0,1], default 0.5. */
+ * eta in (0,1], default 0.9: the last inner run stops close to the outer
+ * target, as the FP64 solve does (measured on three grids: 0.9 needs 0.5 to
+ * 1.5 fewer inner iterations than 0.5 at the same outer count, with an x
+ * error below the FP64 solve's at the same tolerance). */
 int bda_set_refinement(bda_handle h, int32_t adaptive, double eta);
 
 /* Adjoint: solve A^T lambda = w with the transposed colour-ordered ILU0

@@ -223,7 +223,7 @@ class OracleSystem:
         return x, r.as_dict()
 
     def solve_mixed(self, b, x0=None, tol=1e-6, max_outer=20, inner_tol=1e-4, inner_maxit=1000, vec32=True,
-                    adaptive=True, eta=0.5):
+                    adaptive=True, eta=0.9):
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self.N) if x0 is None else np.array(x0, np.float64)
         r = OrcResult()

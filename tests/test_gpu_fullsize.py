@@ -69,3 +69,4 @@ def test_full_mixed_error(big):
     err_f = np.linalg.norm(xf - x64) / np.linalg.norm(x64)
     print(f"grid128x128x64 x error vs 1e-10 FP64 solve: mixed {err_m:.3e}, fp64 {err_f:.3e}")
     assert rm.converged and err_m < 1e-5
+    assert err_m <= err_f  # the precision form is compared with FP64 at equal or better error
