@@ -440,13 +440,10 @@ template <int BS, int MODE> constexpr int stream_cap() { return TileCap<BS, wide
 enum StreamRed : int { SR_NONE = 0, SR_W0 = 1, SR_SELF = 2, SR_BOTH = 3 };
 template <int RED> struct NRedOf { static constexpr int value = (RED & 1) + ((RED >> 1) & 1); };
 
-// Minimum waves per SIMD requested from the register allocator: 8 for the
-// FP32 3x3 product kernel, whose natural allocation lands just above 64
-// VGPRs (7 waves); 1 (no request) elsewhere, where 8 would cause spills.
-template <int BS, int MODE, typename TV, typename TX> struct StreamWaves {
-    static constexpr int value =
-        (std::is_same<TV, float>::value && std::is_same<TX, float>::value && BS == 3 && MODE == SM_SPMV) ? 8 : 1;
-};
+// No occupancy request is made of the register allocator.  The FP32 3x3
+// product kernel lands just above 64 VGPRs (7 waves per SIMD); forcing 8
+// waves was measured slower in the current launch order (54.4-54.5 against
+// 53.8-54.0 us per SpMV on a 128x128x64 grid).
 
 // Phase 1 has no loop: every lane owns a fixed set of S value slots of the
 // tile (S = CAP / blocks per workgroup step) and issues all their loads at
@@ -646,7 +643,7 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
 }
 
 template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
-__global__ __launch_bounds__(64 * BS) __attribute__((amdgpu_waves_per_eu(StreamWaves<BS, MODE, TV, TX>::value))) void k_stream(StreamArgs<TV, TX> a) {
+__global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     // the stop flag and the tile descriptor are loaded together (one scalar
     // round trip instead of two before the first vector load): the exit test
     // also reads the descriptor (a negative entry never occurs), so the
