@@ -126,12 +126,13 @@ struct DevMat {
             const std::vector<int32_t> &o = *orig_row;
             const int64_t n = p.n;
             // key: slice of the layer (an eighth of L), then original row
-            auto key = [&](int t) -> int64_t {
+            std::vector<int64_t> key(nt);
+            for (int t = 0; t < nt; ++t) {
                 const int64_t r = o[tile_row[t]];
                 const int64_t slice = (L >= kLayerSlices && 2 * L <= n) ? ((r % L) * kLayerSlices) / L : 0;
-                return slice * (n + 1) + r;
-            };
-            auto by_key = [&](int a, int b) { return key(a) < key(b); };
+                key[t] = slice * (n + 1) + r;
+            }
+            auto by_key = [&](int a, int b) { return key[a] < key[b]; };
             std::vector<int32_t> ord(nt);
             // whole-matrix list
             for (int t = 0; t < nt; ++t) ord[t] = t;
