@@ -59,6 +59,7 @@ struct SolverState {
     int32_t max_outer;
     int32_t adaptive;
     int32_t inner_h;        // sum of the inner runs' half iterations
+    int32_t full_residual;  // set by k_mark_nonzero: the step-0 residual needs the product
 };
 
 enum Fin : int32_t {
@@ -327,6 +328,19 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts
         }
         finalize(ra.st, ra.fin, t[0], t[1]);
     }
+}
+
+// dst = src (n doubles); *nonfinite = 1 when a value is infinite or NaN.
+// Every writer stores the same value.
+__global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const double *src, double *dst,
+                                                           int32_t *nonfinite) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = src[i];
+        dst[i] = v;
+        bad = bad || !finite(v);
+    }
+    if (bad) __hip_atomic_store(nonfinite, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,7 +656,37 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     }
 }
 
-template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
+// The tile's rows of rhs - A x for a start vector x that is all zero and a
+// matrix whose values are all finite.  The full kernel's products are then
+// signed zeros and its row sum, which starts from +0.0, is +0.0, so
+// rhs - sum is rhs exactly; this path writes the same outputs and the same
+// reduction partials without streaming the matrix.
+template <int BS, int MODE, int RED, typename TV, typename TX>
+__device__ inline void stream_rows_zero_x(const StreamArgs<TV, TX> &a, const int4 d, int slot) {
+    static_assert(MODE == SM_SUB && (RED & SR_W0) == 0, "zero start vector: the plain residual only");
+    constexpr int NT = 64 * BS;
+    constexpr int NRED = NRedOf<RED>::value;
+    const int r0 = d.x, nrow = d.z - d.x;
+    const int t = threadIdx.x;
+    const int k0 = t / BS, r = t - k0 * BS;
+    double red[2] = {0.0, 0.0};
+    if (k0 < nrow) {
+        const size_t gi = (size_t)(r0 + k0) * BS + r;
+        const double sum = 0.0;
+        const TX o = from_double<TX>((double)a.rhs[gi] - sum);
+        a.out[gi] = o;
+        if (a.out2) a.out2[gi] = o;
+        if (a.out3) a.out3[gi] = o;
+        const double od = (double)o;
+        if constexpr (RED == SR_SELF) red[0] = od * od;
+    }
+    if constexpr (NRED == 1) {
+        double v1[1] = {red[0]};
+        block_partials<NT, 1>(v1, a.ra, slot);
+    }
+}
+
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows, bool ZX = false>
 __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     // the stop flag and the tile descriptor are loaded together (one scalar
     // round trip instead of two before the first vector load): the exit test
@@ -654,6 +698,12 @@ __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     // the partial sums go to the tile's own slot, so that the fixed-order sum
     // over them does not depend on the order in which tiles are launched
     const int slot = NRedOf<RED>::value > 0 ? a.slot[tile] : 0;
+    if constexpr (ZX) {
+        if (a.ra.st->full_residual == 0) {
+            stream_rows_zero_x<BS, MODE, RED, TV, TX>(a, d, slot);
+            return;
+        }
+    }
     stream_tile<BS, MODE, RED, TV, TX, ROWS, false>(a, d, nullptr, slot);
 }
 
@@ -795,6 +845,24 @@ __device__ inline VecIdx vec_index(int64_t N) {
         v.second[j] = 2 * p + 1 < N;
     }
     return v;
+}
+
+// Check before a step-0 residual: sets st->full_residual when an entry of x
+// is not zero (a NaN counts as nonzero) or the matrix holds a non-finite
+// value (*nonfinite, recorded when the factorisation copied it).  Every
+// writer stores the same value.
+template <typename TX>
+__global__ __launch_bounds__(kVecThreads) void k_mark_nonzero(int64_t N, const TX *x, const int32_t *nonfinite,
+                                                              SolverState *st) {
+    const VecIdx ix = vec_index(N);
+    bool nz = *nonfinite != 0;
+#pragma unroll
+    for (int j = 0; j < kVecPairsPerThread; ++j) {
+        double a, b;
+        ld2(x, ix.pi[j], a, b);
+        nz = nz || (ix.ok[j] && !(a == 0.0)) || (ix.second[j] && !(b == 0.0));
+    }
+    if (nz) __hip_atomic_store(&st->full_residual, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // steps 6-8 without the x update: s = r - alpha v; ||s||^2.  Step 6,

@@ -276,6 +276,7 @@ struct bda_handle_s {
     // device matrices
     bda::DevMat A, L, U, At, Ut, Lt;
     bda::DBuf<int32_t> d_perm, d_diag, d_blkrow, d_bad;
+    bda::DBuf<int32_t> d_nonfinite;  // set by the factorisation when a matrix value is infinite or NaN
     bda::DBuf<double> d_orig, A64, L64, U64, LU, Dinv, DinvT, At64, Ut64, Lt64;
     bda::DBuf<float> A32, L32, U32;
     uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX;
@@ -395,9 +396,9 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
 
 // Launches the streaming kernel, one workgroup per tile; returns the number
 // of reduction partials written.
-template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows>
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows, bool ZX = false>
 int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
-    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX, ROWS>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX, ROWS, ZX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
     check_launch();
     return a.ntiles;
 }
@@ -405,7 +406,9 @@ int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
 // --- streaming kernel launch over the tiles of one colour (or all) ---------
 // prof_kind >= 0: with profiling on, the streaming kernel alone (not its
 // finalize) is bracketed by an event pair of that kind.
-template <int BS, int MODE, int RED, typename TV, typename TX>
+// ZX: a step-0 residual that takes the short path when k_mark_nonzero left
+// the state's full_residual clear (zero start vector, finite matrix).
+template <int BS, int MODE, int RED, typename TV, typename TX, bool ZX = false>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
                    const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
                    TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
@@ -459,9 +462,9 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     int nparts = 0;
     if constexpr (MODE != SM_SPMV && RED == SR_NONE) {
         if (tall) nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRowsTall>(h, a);
-        else nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+        else nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, ZX>(h, a);
     } else {
-        nparts = stream_dispatch<BS, MODE, RED, TV, TX>(h, a);
+        nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, ZX>(h, a);
     }
     if (prof_kind >= 0) prof_end(h);
     if (nred > 0) launch_finalize(h, nred, a.ra, nparts);
@@ -676,8 +679,13 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         launch_finalize(h, 1, red(h, 0, FIN_INIT), vb);
     } else {
         if constexpr (std::is_same<TX, double>::value) {
-            launch_stream<BS, SM_SUB, SR_SELF, TV, TX>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr, nullptr,
-                                                       red(h, 0, FIN_INIT), true, k.r.p, k.p.p, nullptr, 0, PK_SPMV);
+            // r = b - A x0; with x0 = 0 (and a finite matrix) this is b, without the product
+            hipLaunchKernelGGL((k_mark_nonzero<TX>), dim3(vb), dim3(kVecThreads), 0, h.stream, (int64_t)N,
+                               (const TX *)k.x.p, (const int32_t *)h.d_nonfinite.p, h.st.p);
+            check_launch();
+            launch_stream<BS, SM_SUB, SR_SELF, TV, TX, true>(h, *op.A, op.Aval, -1, k.x.p, rhs, k.rt.p, nullptr,
+                                                             nullptr, red(h, 0, FIN_INIT), true, k.r.p, k.p.p,
+                                                             nullptr, 0, PK_SPMV);
         } else {
             throw std::runtime_error("non-zero start vector needs FP64 vectors");
         }
@@ -828,6 +836,7 @@ void do_analyse(H &h, bool level) {
     h.Dinv.ensure((size_t)h.n * bb);
     h.DinvT.ensure((size_t)h.n * bb);
     h.d_bad.ensure(1);
+    h.d_nonfinite.ensure(1);
     // reduction partials: one slot pair per workgroup of the largest reducing launch
     const int cap = std::max({h.A.ntiles, h.At.ntiles, vec_blocks(h.N()), 1});
     h.partial_cap = cap;
@@ -852,9 +861,16 @@ void do_factor(H &h) {
     if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
     constexpr int BB = BS * BS;
     const Analysis &an = h.an;
-    HIP_CHECK(hipMemcpyAsync(h.LU.p, h.A64.p, (size_t)h.A.nnzb * BB * 8, hipMemcpyDeviceToDevice, h.stream));
-    const int32_t big = INT_MAX;
+    const int32_t big = INT_MAX, zero = 0;
     HIP_CHECK(hipMemcpyAsync(h.d_bad.p, &big, 4, hipMemcpyHostToDevice, h.stream));
+    HIP_CHECK(hipMemcpyAsync(h.d_nonfinite.p, &zero, 4, hipMemcpyHostToDevice, h.stream));
+    {   // LU = A, recording whether A holds a non-finite value (for the step-0 residual's short path)
+        const int64_t nval = (int64_t)h.A.nnzb * BB;
+        const int grid = (int)std::min<int64_t>(std::max<int64_t>(blocks_for(nval, 256), 1), 8192);
+        hipLaunchKernelGGL(k_copy_check_finite, dim3(grid), dim3(256), 0, h.stream, nval, (const double *)h.A64.p,
+                           h.LU.p, h.d_nonfinite.p);
+        check_launch();
+    }
     for (int c = 0; c < an.ncolours; ++c) {
         const int r0 = an.colour_ptr[c], r1 = an.colour_ptr[c + 1];
         hipLaunchKernelGGL((k_bilu0_rows<BS>), dim3(blocks_for(r1 - r0, 128)), dim3(128), 0, h.stream, r0, r1,
@@ -987,9 +1003,18 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     check_launch();
     bool aborted = false;
     for (int outer = 0;; ++outer) {
-        // FP64 true residual ro = b - A x and the outer step's decisions
-        launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
-                                                           nullptr, red(h, 1, FIN_REFINE), true);
+        // FP64 true residual ro = b - A x and the outer step's decisions; the
+        // first one is b without the product when x = 0 and A is finite
+        if (outer == 0) {
+            hipLaunchKernelGGL((k_mark_nonzero<double>), dim3(vec_blocks(h.N())), dim3(kVecThreads), 0, h.stream,
+                               (int64_t)h.N(), (const double *)h.xo.p, (const int32_t *)h.d_nonfinite.p, h.st.p + 1);
+            check_launch();
+            launch_stream<BS, SM_SUB, SR_SELF, double, double, true>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p,
+                                                                     nullptr, nullptr, red(h, 1, FIN_REFINE), true);
+        } else {
+            launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
+                                                               nullptr, red(h, 1, FIN_REFINE), true);
+        }
         if (outer == max_outer) break;
         if (!wait_progress(h, 1, os.run_id, outer + 1, t0)) {
             aborted = true;
