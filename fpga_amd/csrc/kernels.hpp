@@ -4,7 +4,8 @@
 //
 // Kernel map (reference op -> kernel):
 //   SpMV mode, sparstition.vhd:655-878 / spmvp.vhd:240-383  -> k_stream<SPMV>
-//   steps 0-1 (r = b - A x0, rho)  solver_pkg.vhd:122-123     -> k_stream<SUB> + FIN_INIT
+//   steps 0-1 (r = b - A x0, rho)  solver_pkg.vhd:122-123     -> k_mark_nonzero + k_stream<SUB, ZX> + FIN_INIT
+//                                                                 (r = b without the product when x0 = 0)
 //   ILU0_L_fs, ilu0.vhd:412-416                               -> k_stream<SUB> per colour
 //   ILU0_U_bs, ilu0.vhd:385-410                               -> k_stream<SUB_DINV> per colour
 //   dot1 -> alpha, solver.vhd:793-798                         -> SpMV epilogue + FIN_ALPHA
