@@ -692,12 +692,16 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     }
     // Iterations are queued ahead of the device; the device stops itself
     // (every kernel returns at once after the stop) and raises a mapped host
-    // flag.  The host keeps at most 2 * ahead - 1 half-iterations queued
-    // beyond the one running (ahead = bda_set_options' poll_iters, 1..3): it
-    // waits on the mapped progress word (no stream event), which the scalar
-    // step of every half-iteration advances, and checks the stop flag each
-    // time.  What is still queued when the device stops runs as no-op
-    // launches (a few microseconds each), so the queue is kept short.
+    // flag.  The host checks its lead in the middle of every half-iteration,
+    // after queueing that half's preconditioner: it waits until at most
+    // 2 * ahead - 1 earlier half-iterations are unfinished (ahead =
+    // bda_set_options' poll_iters, 1..3), on the mapped progress word (no
+    // stream event), which the scalar step of every half-iteration advances,
+    // and checks the stop flag each time.  What is still queued when the
+    // device stops runs as no-op launches (a few microseconds each); with
+    // the check in mid half-iteration that is the preconditioner's launches
+    // of one half at most (ahead = 1), while the device still has that
+    // preconditioner queued whenever the host waits.
     bool stop = false;
     const int ahead = std::min(3, std::max(1, h.poll_iters));
     const int ahead_h = 2 * ahead - 1;
@@ -718,6 +722,8 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     for (int it = 1; it <= maxit + 1 && !stop; ++it) {
         h.prof_tag = 2 * (it - 1);
         precond<BS, TV, TX>(h, op, k.p.p, k.y.p, k.phat.p, true);
+        throttle(2 * it - 1);
+        if (stop) break;
         launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, *op.A, op.Aval, -1, k.phat.p, nullptr, k.v.p, nullptr, k.rt.p,
                                                   red(h, 0, FIN_ALPHA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
@@ -726,10 +732,10 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
         check_launch();
         launch_finalize(h, 1, red(h, 0, FIN_HALF1), vb);
         prof_end(h);
-        throttle(2 * it - 1);
-        if (stop) break;
         h.prof_tag = 2 * (it - 1) + 1;
         precond<BS, TV, TX>(h, op, k.s.p, k.y.p, k.shat.p, true);
+        throttle(2 * it);
+        if (stop) break;
         launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, *op.A, op.Aval, -1, k.shat.p, nullptr, k.t.p, nullptr, k.s.p,
                                                     red(h, 0, FIN_OMEGA), true, nullptr, nullptr, nullptr, 0, PK_SPMV);
         prof_begin(h, PK_VEC);
@@ -741,7 +747,6 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
                            k.v.p, (const SolverState *)(h.st.p));
         check_launch();
         prof_end(h);
-        throttle(2 * it);
     }
     // x += alpha phat of a run that ended after a first half-iteration (the
     // x update of a first half is otherwise applied by the next k_update_xr)
