@@ -159,8 +159,9 @@ int bda_factor(bda_handle h);
  * (bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:605-643); abs_tol != 0
  * selects the absolute compare of config_bits bit 1 (solver.vhd:637-641);
  * poll_iters (1..3) sets how far the host queues ahead of the device, which
- * stops itself and raises a host-visible flag: 2 * poll_iters - 1
- * half-iterations beyond the running one. */
+ * stops itself and raises a host-visible flag: in the middle of each
+ * half-iteration the host waits until at most 2 * poll_iters - 1 earlier
+ * half-iterations are unfinished. */
 int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t poll_iters);
 
 /* Plain FP64 ILU0-BiCGSTAB, the reference's microprogram
