@@ -699,6 +699,7 @@ __global__ __launch_bounds__(64 * BS) void k_stream(StreamArgs<TV, TX> a) {
     // the partial sums go to the tile's own slot, so that the fixed-order sum
     // over them does not depend on the order in which tiles are launched
     const int slot = NRedOf<RED>::value > 0 ? a.slot[tile] : 0;
+    static_assert(!ZX || ROWS == 64, "the zero-start path serves one row per thread");
     if constexpr (ZX) {
         if (a.ra.st->full_residual == 0) {
             stream_rows_zero_x<BS, MODE, RED, TV, TX>(a, d, slot);
