@@ -12,10 +12,17 @@ import fpga_amd as fa
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def handle():
-    print("case grid64x64x32 zero start", flush=True)
-    s = fa.generate(fa.configs.GRID_64x64x32)
+SPECS = {
+    "grid64x64x32": fa.configs.GRID_64x64x32,
+    "grid48x48x24_bs4_nnc": fa.GridSpec(48, 48, 24, bs=4, seed=2, perm_sigma=2.0, layer_corr=0.5,
+                                        inactive_frac=0.10, nnc_frac=0.01, nnc_dist=3),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(SPECS))
+def handle(request):
+    print(f"case {request.param} zero start", flush=True)
+    s = fa.generate(SPECS[request.param])
     g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
     g.upload_system(s)
     g.analyse()
