@@ -54,3 +54,29 @@ def test_mixed_solve_repeats_bit_identical(system):
         np.testing.assert_array_equal(xa, xb)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("poll_iters", [2, 3])
+def test_host_lead_does_not_change_results(system, poll_iters):
+    """How far the host queues ahead (bda_set_options' poll_iters) changes
+    only how many launches are pending, never the results, including the
+    no-op launches left queued when a run stops in either half-iteration."""
+    s = system
+    g = make(s)
+    try:
+        x1, r1 = g.solve(s.b, tol=1e-8, maxit=500)
+        xm1, rm1 = g.solve_mixed(s.b, tol=1e-8)
+        g.set_options(poll_iters=poll_iters)
+        x2, r2 = g.solve(s.b, tol=1e-8, maxit=500)
+        xm2, rm2 = g.solve_mixed(s.b, tol=1e-8)
+        x3, r3 = g.solve(s.b, tol=0.0, maxit=7)  # stops at a first half (max-iteration exit)
+        g.set_options(poll_iters=1)
+        x4, r4 = g.solve(s.b, tol=0.0, maxit=7)
+        assert r1.half_iterations == r2.half_iterations
+        np.testing.assert_array_equal(x1, x2)
+        assert rm1.inner_half_iterations == rm2.inner_half_iterations
+        np.testing.assert_array_equal(xm1, xm2)
+        assert r3.max_iter_reached and r3.half_iterations == r4.half_iterations == 14
+        np.testing.assert_array_equal(x3, x4)
+    finally:
+        g.close()
