@@ -85,6 +85,7 @@ SIGNATURES = {
     "bda_upload_bcsr": (i32, [vp, vp, vp, vp]),
     "bda_update_values": (i32, [vp, vp]),
     "bda_update_values_dev": (i32, [vp, vp]),
+    "bda_values_generation": (i32, [vp, C.POINTER(C.c_uint64)]),
     "bda_analyse": (i32, [vp, i32]),
     "bda_num_colours": (i32, [vp, P_i32]),
     "bda_get_colouring": (i32, [vp, vp, vp, vp]),
@@ -102,6 +103,8 @@ SIGNATURES = {
     "bda_grad_values_dev": (i32, [vp, vp, vp, vp]),
     "bda_spmv_perm": (i32, [vp, vp, vp, i32]),
     "bda_precond_apply_perm": (i32, [vp, vp, vp, i32]),
+    "bda_spmv_perm_ex": (i32, [vp, i32, vp, vp, i32, vp, i32, P_f64]),
+    "bda_precond_apply_perm_ex": (i32, [vp, i32, vp, vp, i32]),
     "bda_factor_sizes": (i32, [vp, P_i32, P_i32]),
     "bda_get_factors": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
     "bda_set_profiling": (i32, [vp, i32]),

@@ -61,6 +61,13 @@ struct SolverState {
     int32_t adaptive;
     int32_t inner_h;        // sum of the inner runs' half iterations
     int32_t full_residual;  // set by k_mark_nonzero: the step-0 residual needs the product
+    // watchdog (the counterpart of the reference kernel's abort cycle count,
+    // param0): timeout in ticks of the device's constant-rate wall clock
+    // (0 = off); the run's deadline is set on the device when its state is
+    // initialised, and every scalar step past it stops the run as aborted
+    uint64_t timeout_ticks;
+    uint64_t deadline;
+    int32_t aborted;
 };
 
 enum Fin : int32_t {
@@ -117,8 +124,15 @@ __device__ inline void signal_progress(SolverState *st, int32_t count) {
                            __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Scalar epilogues; run by one thread after a grid-wide reduction.
+// Scalar epilogues; run by one thread after a grid-wide reduction.  The
+// watchdog is checked first: a run past its deadline stops here, and every
+// launch queued after it returns at once.
 __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) {
+    if (fin != FIN_STORE && st->deadline != 0 && wall_clock64() > st->deadline) {
+        st->aborted = 1;
+        signal_done(st);
+        return;
+    }
     switch (fin) {
     case FIN_STORE:
         st->tot[0] = t0;
@@ -227,12 +241,18 @@ __device__ inline void finalize(SolverState *st, int fin, double t0, double t1) 
     }
 }
 
-// Solver state set-up on the stream (no host copy, no synchronisation).
-__global__ void k_set_state(SolverState *st, SolverState v) { *st = v; }
+// Solver state set-up on the stream (no host copy, no synchronisation); the
+// watchdog's deadline counts from here.
+__global__ void k_set_state(SolverState *st, SolverState v) {
+    if (v.timeout_ticks != 0) v.deadline = wall_clock64() + v.timeout_ticks;
+    *st = v;
+}
 
-// Start of an inner run of the precision form: tolerance from the outer
-// state; an outer state that has already stopped stops the inner run.
+// Start of an inner run of the precision form: tolerance and deadline from
+// the outer state; an outer state that has already stopped stops the inner
+// run.
 __global__ void k_inner_start(SolverState *in, const SolverState *out, SolverState v) {
+    v.deadline = out->deadline;
     *in = v;
     if (out->done) {
         in->no_results = 1;
@@ -243,7 +263,9 @@ __global__ void k_inner_start(SolverState *in, const SolverState *out, SolverSta
 }
 
 // End of an inner run: counters and flags into the outer state.  An inner
-// run whose right-hand side was already zero ends the refinement.
+// run whose right-hand side was already zero ends the refinement, and so
+// does one that broke down or met a NaN before completing a half-iteration:
+// it changed nothing, so the next outer step would repeat it exactly.
 __global__ void k_refine_step(SolverState *out, const SolverState *in) {
     if (out->done) return;
     out->outer += 1;
@@ -253,7 +275,9 @@ __global__ void k_refine_step(SolverState *out, const SolverState *in) {
         out->breakdown = in->breakdown;
         out->nan_flag = in->nan_flag;
     }
-    if (in->no_results) signal_done(out);
+    if (in->aborted) out->aborted = 1;
+    if (in->no_results || in->aborted || ((in->breakdown || in->nan_flag) && in->h == 0 && in->last_half == 0))
+        signal_done(out);
 }
 
 __device__ inline double wave_sum(double v) {
@@ -381,6 +405,11 @@ struct StreamArgs {
     const double *dinv; // SUB_DINV: out = dinv * (rhs - A x)
     const TX *w0;       // SR_W0 reductions: sum out * w0
     RedArgs ra;
+    // SUB_DINV launch that runs both sweeps of a colour at once (the first
+    // sweep's last colour, whose rows have no second-sweep blocks): the first
+    // sweep's result rhs - A x is rounded to TX, as the vector the second
+    // sweep would read, before D^-1 is applied
+    int32_t round_mid;
 };
 
 template <typename TV, typename TX>
@@ -616,6 +645,12 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
         }
     }
     if constexpr (MODE == SM_SUB_DINV) {
+        if constexpr (!std::is_same<TX, double>::value) {
+            if (a.round_mid) {
+#pragma unroll
+                for (int s = 0; s < RS; ++s) res[s] = (double)from_double<TX>(res[s]);
+            }
+        }
 #pragma unroll
         for (int s = 0; s < RS; ++s) tv[s * NT + t] = res[s];
         __syncthreads();

@@ -264,6 +264,7 @@ struct bda_handle_s {
     int64_t nnzb_max = 0;
     int device = 0;
     int cus = 0;  // compute units of the device
+    int wall_khz = 0;  // rate of the device's wall clock (the watchdog's time base)
     hipStream_t stream = nullptr;
     std::string err;
 
@@ -279,12 +280,14 @@ struct bda_handle_s {
     bda::DBuf<int32_t> d_nonfinite;  // set by the factorisation when a matrix value is infinite or NaN
     bda::DBuf<double> d_orig, A64, L64, U64, LU, Dinv, DinvT, At64, Ut64, Lt64;
     bda::DBuf<float> A32, L32, U32;
-    uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX;
+    bda::DBuf<float> At32, Ut32, Lt32;  // FP32 transposes (component access only)
+    uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX, adj32_gen = UINT64_MAX;
+    uint64_t values_gen = 0;  // changes with every upload, value update and factorisation
 
     // vectors
     bda::Krylov<double> k64;
     bda::Krylov<float> k32;
-    bda::DBuf<double> bp, xo, ro, vin, vout, gl, gx, ggrad;
+    bda::DBuf<double> bp, xo, ro, vin, vout, vw, gl, gx, ggrad;
     bool last_solve_valid = false;
     int last_kind = 0;  // 0 plain / adjoint on k64, 1 mixed (residual in ro)
 
@@ -411,7 +414,8 @@ int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
 template <int BS, int MODE, int RED, typename TV, typename TX, bool ZX = false>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
                    const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
-                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
+                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1,
+                   bool round_mid = false) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
     constexpr bool wide = wide_tiles<MODE>();
@@ -454,6 +458,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     a.dinv = dinv;
     a.w0 = w0;
     a.ra = ra;
+    a.round_mid = round_mid ? 1 : 0;
     if (!use_state) a.ra.st = h.st.p + kIdleState;
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
@@ -496,11 +501,11 @@ int vec_blocks(size_t N) { return (int)((N + kVecElemsPerBlock - 1) / kVecElemsP
 
 template <int BS, typename TV, typename TX>
 void sweep_launch(H &h, const DevMat &m, const TV *val, int c, bool dinv_mode, const TX *x, const TX *x_lo, int split,
-                  const TX *rhs, TX *out, const double *dinv, bool use_state) {
+                  const TX *rhs, TX *out, const double *dinv, bool use_state, bool round_mid = false) {
     RedArgs ra = red(h, 0, FIN_NONE);
     if (dinv_mode)
         launch_stream<BS, SM_SUB_DINV, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, dinv, nullptr, ra, use_state,
-                                                        nullptr, nullptr, x_lo, split);
+                                                        nullptr, nullptr, x_lo, split, -1, round_mid);
     else
         launch_stream<BS, SM_SUB, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, nullptr, nullptr, ra, use_state, nullptr,
                                                    nullptr, x_lo, split);
@@ -539,6 +544,7 @@ void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split
     a.out = z;
     a.dinv = op.dinv;
     a.ra = red(h, 0, FIN_NONE);
+    a.round_mid = op.s2_dinv ? 1 : 0;  // the first sweep's rows of colour C-1 rounded before the second's D^-1
     if (!use_state) a.ra.st = h.st.p + kIdleState;
     TailArgs<TV, TX> b;
     b.rowptr = m2.rowptr.p;
@@ -585,7 +591,7 @@ void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_stat
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
     if (!merge)
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
-                                 use_state);
+                                 use_state, op.s2_dinv);
     prof_end(h);
     prof_begin(h, PK_UPPER);
     if (merge) tail_launch<BS, TV, TX>(h, op, y, lo, split, p, z, use_state);
@@ -615,6 +621,8 @@ SolverState new_state(H &h, int idx, double tol, int maxit, bool abs_mode) {
     s.run_id = h.run_seq;
     s.host_done = h.d_done + idx;
     s.host_prog = h.d_prog + idx;
+    if (h.timeout_s > 0.0)
+        s.timeout_ticks = (uint64_t)std::max(1.0, std::ceil(h.timeout_s * 1e3 * (double)h.wall_khz));
     if (idx == 0 && h.hist_cap > 0) {
         s.hist = h.hist.p;
         s.hist_cap = h.hist_cap;
@@ -781,7 +789,7 @@ void fill_result(const RunOut &o, bda_result *res) {
     res->max_iter_reached = s.max_iter;
     res->breakdown = s.breakdown;
     res->nan_detected = s.nan_flag;
-    res->aborted = o.aborted ? 1 : 0;
+    res->aborted = (o.aborted || s.aborted) ? 1 : 0;
     res->solve_ms = o.ms;
 }
 
@@ -852,6 +860,7 @@ void do_analyse(H &h, bool level) {
     h.ro.ensure(h.Npad());
     h.vin.ensure(h.Npad());
     h.vout.ensure(h.Npad());
+    h.vw.ensure(h.Npad());
     h.analysed = true;
     h.factored = false;
     if (h.have_values) {
@@ -894,6 +903,7 @@ void do_factor(H &h) {
     HIP_CHECK(hipStreamSynchronize(h.stream));
     h.factored = true;
     h.factor_gen++;
+    h.values_gen++;
 }
 
 template <int BS>
@@ -926,6 +936,24 @@ void ensure_adjoint(H &h) {
     h.adj_gen = h.factor_gen;
 }
 
+// FP32 copies of the transposed operators, demoted from the FP64 transposes
+// (the same values the oracle's FP32 transposes hold).  Only the component
+// access uses them: the adjoint solve runs in FP64.
+template <int BS>
+void ensure_adjoint32(H &h) {
+    ensure_adjoint<BS>(h);
+    if (h.adj32_gen == h.factor_gen) return;
+    const size_t bb = (size_t)BS * BS;
+    h.At32.ensure((h.At.nnzb + kPadBlocks) * bb);
+    h.Ut32.ensure((h.Ut.nnzb + kPadBlocks) * bb);
+    h.Lt32.ensure((h.Lt.nnzb + kPadBlocks) * bb);
+    gather_blocks<BS, float>(h, h.At.nnzb, nullptr, h.At64.p, h.At32.p, 0);
+    gather_blocks<BS, float>(h, h.Ut.nnzb, nullptr, h.Ut64.p, h.Ut32.p, 0);
+    gather_blocks<BS, float>(h, h.Lt.nnzb, nullptr, h.Lt64.p, h.Lt32.p, 0);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.adj32_gen = h.factor_gen;
+}
+
 OpSet<double> op_fwd64(H &h) {
     return OpSet<double>{&h.A, h.A64.p, &h.L, h.L64.p, false, &h.U, h.U64.p, true, h.Dinv.p};
 }
@@ -934,6 +962,9 @@ OpSet<float> op_fwd32(H &h) {
 }
 OpSet<double> op_adj64(H &h) {
     return OpSet<double>{&h.At, h.At64.p, &h.Ut, h.Ut64.p, true, &h.Lt, h.Lt64.p, false, h.DinvT.p};
+}
+OpSet<float> op_adj32(H &h) {
+    return OpSet<float>{&h.At, h.At32.p, &h.Ut, h.Ut32.p, true, &h.Lt, h.Lt32.p, false, h.DinvT.p};
 }
 
 void gather_vec(H &h, const double *in, double *out) {
@@ -1053,7 +1084,7 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     r.max_iter_reached = s.max_iter;
     r.breakdown = s.breakdown;
     r.nan_detected = s.nan_flag;
-    r.aborted = aborted ? 1 : 0;
+    r.aborted = (aborted || s.aborted) ? 1 : 0;
     r.outer_iterations = s.outer;
     r.inner_half_iterations = s.inner_h;
     r.half_iterations = s.inner_h;
@@ -1065,36 +1096,98 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     h.last_kind = 1;
 }
 
-template <int BS>
-void spmv_perm(H &h, const double *x, double *y, int transpose) {
-    require_analysed(h);
-    if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
-    if (transpose) {
-        require_factored(h);
-        ensure_adjoint<BS>(h);
+// Component access in the three kernel forms of the solves:
+//   PREC_F64      FP64 values and vectors (bda_solve, the adjoint)
+//   PREC_F32      FP32 values and vectors (bda_solve_mixed with vec32 != 0)
+//   PREC_F32_X64  FP32 values, FP64 vectors (bda_solve_mixed with vec32 = 0)
+enum Prec : int { PREC_F64 = 0, PREC_F32 = 1, PREC_F32_X64 = 2 };
+
+// y = A x (or A^T x) with the solve's SpMV kernel instance; redm = 1 or 3
+// launches the instance with the fused reductions the solve uses
+// (v = A phat with v.rt; t = A shat with t.s and t.t), summed in the
+// solve's fixed order into dots[].  Vectors are device buffers of the
+// handle in the colour-ordered numbering.
+template <int BS, typename TV, typename TX>
+void spmv_comp(H &h, const TX *x, TX *y, int transpose, const TX *w, int redm, double *dots) {
+    const DevMat &m = transpose ? h.At : h.A;
+    const TV *val;
+    if constexpr (std::is_same<TV, float>::value) val = transpose ? h.At32.p : h.A32.p;
+    else val = transpose ? h.At64.p : h.A64.p;
+    if (redm == 0) {
+        launch_stream<BS, SM_SPMV, SR_NONE, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, nullptr,
+                                                    red(h, 1, FIN_NONE), false);
+    } else {
+        init_state(h, 1, 0.0, 0, false);
+        if (redm == 1)
+            launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w,
+                                                      red(h, 1, FIN_STORE), true);
+        else
+            launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w,
+                                                        red(h, 1, FIN_STORE), true);
     }
-    HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * 8, hipMemcpyHostToDevice));
-    RedArgs ra = red(h, 1, FIN_NONE);
-    if (transpose)
-        launch_stream<BS, SM_SPMV, SR_NONE, double, double>(h, h.At, h.At64.p, -1, h.vin.p, nullptr, h.vout.p,
-                                                            nullptr, nullptr, ra, false);
-    else
-        launch_stream<BS, SM_SPMV, SR_NONE, double, double>(h, h.A, h.A64.p, -1, h.vin.p, nullptr, h.vout.p, nullptr,
-                                                            nullptr, ra, false);
     HIP_CHECK(hipStreamSynchronize(h.stream));
-    HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * 8, hipMemcpyDeviceToHost));
+    if (redm != 0 && dots) {
+        SolverState s;
+        HIP_CHECK(hipMemcpy(&s, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost));
+        dots[0] = s.tot[0];
+        dots[1] = redm == 3 ? s.tot[1] : 0.0;
+    }
 }
 
 template <int BS>
-void precond_perm(H &h, const double *p, double *z, int transpose) {
+void comp_prepare(H &h, int prec, int transpose) {
+    require_analysed(h);
+    if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
+    if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_F32_X64)
+        throw std::runtime_error("precision must be 0 (FP64), 1 (FP32 vectors) or 2 (FP64 vectors)");
+    if (prec != PREC_F64) {
+        ensure_mixed<BS>(h);  // the FP32 copies are made from the factored system
+        if (transpose) ensure_adjoint32<BS>(h);
+    } else if (transpose) {
+        require_factored(h);
+        ensure_adjoint<BS>(h);
+    }
+}
+
+template <int BS>
+void spmv_perm(H &h, int prec, const void *x, void *y, int transpose, const void *w, int redm, double *dots) {
+    comp_prepare<BS>(h, prec, transpose);
+    if (redm != 0 && redm != 1 && redm != 3) throw std::runtime_error("red must be 0, 1 or 3");
+    if (redm != 0 && transpose) throw std::runtime_error("fused reductions: forward operator only");
+    if (redm != 0 && !w) throw std::runtime_error("the fused reductions need w");
+    const size_t es = prec == PREC_F32 ? 4 : 8;
+    HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * es, hipMemcpyHostToDevice));
+    if (w) HIP_CHECK(hipMemcpy(h.vw.p, w, h.N() * es, hipMemcpyHostToDevice));
+    if (prec == PREC_F32)
+        spmv_comp<BS, float, float>(h, (const float *)h.vin.p, (float *)h.vout.p, transpose, (const float *)h.vw.p,
+                                    redm, dots);
+    else if (prec == PREC_F32_X64)
+        spmv_comp<BS, float, double>(h, h.vin.p, h.vout.p, transpose, h.vw.p, redm, dots);
+    else
+        spmv_comp<BS, double, double>(h, h.vin.p, h.vout.p, transpose, h.vw.p, redm, dots);
+    HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * es, hipMemcpyDeviceToHost));
+}
+
+template <int BS>
+void precond_perm(H &h, int prec, const void *p, void *z, int transpose) {
     require_factored(h);
-    if (transpose) ensure_adjoint<BS>(h);
-    HIP_CHECK(hipMemcpy(h.vin.p, p, h.N() * 8, hipMemcpyHostToDevice));
-    OpSet<double> op = transpose ? op_adj64(h) : op_fwd64(h);
-    precond<BS, double, double>(h, op, h.vin.p, h.k64.y.p, h.vout.p, false);
+    comp_prepare<BS>(h, prec, transpose);
+    const size_t es = prec == PREC_F32 ? 4 : 8;
+    HIP_CHECK(hipMemcpy(h.vin.p, p, h.N() * es, hipMemcpyHostToDevice));
+    // the intermediate of the two sweeps goes to vw
+    if (prec == PREC_F32) {
+        OpSet<float> op = transpose ? op_adj32(h) : op_fwd32(h);
+        precond<BS, float, float>(h, op, (const float *)h.vin.p, (float *)h.vw.p, (float *)h.vout.p, false);
+    } else if (prec == PREC_F32_X64) {
+        OpSet<float> op = transpose ? op_adj32(h) : op_fwd32(h);
+        precond<BS, float, double>(h, op, h.vin.p, h.vw.p, h.vout.p, false);
+    } else {
+        OpSet<double> op = transpose ? op_adj64(h) : op_fwd64(h);
+        precond<BS, double, double>(h, op, h.vin.p, h.vw.p, h.vout.p, false);
+    }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     prof_collect(h);
-    HIP_CHECK(hipMemcpy(z, h.vout.p, h.N() * 8, hipMemcpyDeviceToHost));
+    HIP_CHECK(hipMemcpy(z, h.vout.p, h.N() * es, hipMemcpyDeviceToHost));
 }
 
 template <int BS>
@@ -1185,6 +1278,8 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
     try {
         HIP_CHECK(hipGetDevice(&h->device));
         HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
+        HIP_CHECK(hipDeviceGetAttribute(&h->wall_khz, hipDeviceAttributeWallClockRate, h->device));
+        if (h->wall_khz <= 0) throw std::runtime_error("device reports no wall clock rate");
         // a blocking stream: it orders itself after work the caller queued on
         // the legacy default stream (torch's default), so device inputs
         // produced there are complete before the handle reads them
@@ -1214,10 +1309,10 @@ int bda_destroy(bda_handle h) {
     for (DevMat *m : {&h->A, &h->L, &h->U, &h->At, &h->Ut, &h->Lt}) m->release();
     for (DBuf<int32_t> *b : {&h->d_perm, &h->d_diag, &h->d_blkrow, &h->d_bad}) b->release();
     for (DBuf<double> *b : {&h->d_orig, &h->A64, &h->L64, &h->U64, &h->LU, &h->Dinv, &h->DinvT, &h->At64, &h->Ut64,
-                            &h->Lt64, &h->bp, &h->xo, &h->ro, &h->vin, &h->vout, &h->gl, &h->gx, &h->ggrad,
+                            &h->Lt64, &h->bp, &h->xo, &h->ro, &h->vin, &h->vout, &h->vw, &h->gl, &h->gx, &h->ggrad,
                             &h->partials})
         b->release();
-    for (DBuf<float> *b : {&h->A32, &h->L32, &h->U32}) b->release();
+    for (DBuf<float> *b : {&h->A32, &h->L32, &h->U32, &h->At32, &h->Ut32, &h->Lt32}) b->release();
     h->k64.release();
     h->k32.release();
     h->st.release();
@@ -1253,11 +1348,18 @@ int bda_query(bda_handle h, bda_limits *lim) {
 int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx, const double *values) {
     return guarded(h, [&] {
         if (!rowptr || !colidx || !values) throw std::runtime_error("null array");
-        const int64_t nnzb = rowptr[h->n];
-        if (nnzb > h->nnzb_max)
-            throw std::runtime_error("nnzb " + std::to_string(nnzb) + " exceeds nnzb_max " +
-                                     std::to_string(h->nnzb_max));
+        // the pattern is checked before anything is copied: rowptr
+        // non-decreasing from 0, within nnzb_max, and every column in range
         if (rowptr[0] != 0) throw std::runtime_error("rowptr[0] must be 0");
+        for (int i = 0; i < h->n; ++i)
+            if (rowptr[i + 1] < rowptr[i] || rowptr[i + 1] > h->nnzb_max)
+                throw std::runtime_error("rowptr[" + std::to_string(i + 1) + "] = " + std::to_string(rowptr[i + 1]) +
+                                         " is decreasing or exceeds nnzb_max " + std::to_string(h->nnzb_max));
+        const int64_t nnzb = rowptr[h->n];
+        for (int64_t q = 0; q < nnzb; ++q)
+            if (colidx[q] < 0 || colidx[q] >= h->n)
+                throw std::runtime_error("colidx[" + std::to_string(q) + "] = " + std::to_string(colidx[q]) +
+                                         " out of range");
         h->pattern.bs = h->bs;
         h->pattern.n = h->n;
         h->pattern.rowptr.assign(rowptr, rowptr + h->n + 1);
@@ -1267,6 +1369,7 @@ int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx, 
         h->factored = false;
         HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
         h->have_values = true;
+        h->values_gen++;
     });
 }
 
@@ -1277,6 +1380,7 @@ int bda_update_values(bda_handle h, const double *values) {
         HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
         h->have_values = true;
         h->factored = false;
+        h->values_gen++;
         if (h->analysed) BS_CALL(h, permute_values, *h);
     });
 }
@@ -1288,7 +1392,15 @@ int bda_update_values_dev(bda_handle h, const double *d_values) {
         HIP_CHECK(hipMemcpy(h->d_orig.p, d_values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyDeviceToDevice));
         h->have_values = true;
         h->factored = false;
+        h->values_gen++;
         if (h->analysed) BS_CALL(h, permute_values, *h);
+    });
+}
+
+int bda_values_generation(bda_handle h, uint64_t *gen) {
+    return guarded(h, [&] {
+        if (!gen) throw std::runtime_error("null pointer");
+        *gen = h->values_gen;
     });
 }
 
@@ -1417,11 +1529,32 @@ int bda_grad_values(bda_handle h, const double *lambda, const double *x, double 
 }
 
 int bda_spmv_perm(bda_handle h, const double *x, double *y, int32_t transpose) {
-    return guarded(h, [&] { BS_CALL(h, spmv_perm, *h, x, y, transpose); });
+    return guarded(h, [&] {
+        if (!x || !y) throw std::runtime_error("null vector");
+        BS_CALL(h, spmv_perm, *h, PREC_F64, x, y, transpose, nullptr, 0, nullptr);
+    });
 }
 
 int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t transpose) {
-    return guarded(h, [&] { BS_CALL(h, precond_perm, *h, p, z, transpose); });
+    return guarded(h, [&] {
+        if (!p || !z) throw std::runtime_error("null vector");
+        BS_CALL(h, precond_perm, *h, PREC_F64, p, z, transpose);
+    });
+}
+
+int bda_spmv_perm_ex(bda_handle h, int32_t prec, const void *x, void *y, int32_t transpose, const void *w,
+                     int32_t red, double dots[2]) {
+    return guarded(h, [&] {
+        if (!x || !y) throw std::runtime_error("null vector");
+        BS_CALL(h, spmv_perm, *h, prec, x, y, transpose, w, red, dots);
+    });
+}
+
+int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z, int32_t transpose) {
+    return guarded(h, [&] {
+        if (!p || !z) throw std::runtime_error("null vector");
+        BS_CALL(h, precond_perm, *h, prec, p, z, transpose);
+    });
 }
 
 int bda_factor_sizes(bda_handle h, int32_t *nl, int32_t *nu) {

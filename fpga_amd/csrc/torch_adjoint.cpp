@@ -57,8 +57,11 @@ struct BdaSolve : public torch::autograd::Function<BdaSolve> {
         check(h, bda_solve_dev(h, b.data_ptr<double>(), x.data_ptr<double>(), tol, (int32_t)maxit, &res),
               "bda_solve_dev");
         TORCH_CHECK(res.converged || res.max_iter_reached, "forward solve failed (breakdown or NaN)");
-        ctx->save_for_backward({x});
+        uint64_t gen = 0;
+        check(h, bda_values_generation(h, &gen), "bda_values_generation");
+        ctx->save_for_backward({x, values});
         ctx->saved_data["handle"] = handle;
+        ctx->saved_data["generation"] = (int64_t)gen;
         ctx->saved_data["adj_tol"] = adj_tol;
         ctx->saved_data["adj_maxit"] = adj_maxit;
         ctx->saved_data["nvalues"] = values.numel();
@@ -68,11 +71,21 @@ struct BdaSolve : public torch::autograd::Function<BdaSolve> {
 
     static tensor_list backward(AutogradContext *ctx, tensor_list grad_out) {
         bda_handle h = as_handle(ctx->saved_data["handle"].toInt());
-        const Tensor x = ctx->get_saved_variables()[0];
+        const auto saved = ctx->get_saved_variables();
+        const Tensor x = saved[0], values = saved[1];
         Tensor w = grad_out[0].contiguous().to(torch::kFloat64);
         check_tensor(w, "grad_output");
         Tensor lam = torch::zeros_like(w);
         torch::cuda::synchronize();
+        // The handle may hold another system by now (a second forward call,
+        // a value update or a factorisation since this forward): the adjoint
+        // must use the values this forward solved with, so they are put back.
+        uint64_t gen = 0;
+        check(h, bda_values_generation(h, &gen), "bda_values_generation");
+        if ((int64_t)gen != ctx->saved_data["generation"].toInt()) {
+            check(h, bda_update_values_dev(h, values.data_ptr<double>()), "bda_update_values_dev");
+            check(h, bda_factor(h), "bda_factor");
+        }
         bda_result res;
         check(h,
               bda_adjoint_solve_dev(h, w.data_ptr<double>(), lam.data_ptr<double>(),

@@ -212,6 +212,12 @@ class BicgstabSolver:
     def update_values_dev(self, ptr: int):
         self._check(lib.bda_update_values_dev(self._h, C.c_void_p(ptr)), "bda_update_values_dev")
 
+    @property
+    def values_generation(self) -> int:
+        g = C.c_uint64()
+        self._check(lib.bda_values_generation(self._h, C.byref(g)), "bda_values_generation")
+        return g.value
+
     def analyse(self, level_scheduling: bool = False):
         self._check(lib.bda_analyse(self._h, int(level_scheduling)), "bda_analyse")
 
@@ -309,6 +315,29 @@ class BicgstabSolver:
         p = _f64(p)
         z = np.empty(self.N)
         self._check(lib.bda_precond_apply_perm(self._h, _p(p), _p(z), int(transpose)), "bda_precond_apply_perm")
+        return z
+
+    _PREC_DTYPE = {0: np.float64, 1: np.float32, 2: np.float64}
+
+    def spmv_perm_ex(self, x, prec: int = 0, transpose: bool = False, w=None, red: int = 0):
+        """y = A x in kernel form ``prec`` (0 FP64, 1 FP32 values and vectors,
+        2 FP32 values with FP64 vectors); red = 1 or 3 also returns the fused
+        dots (y.w; y.w and y.y) of the solve's SpMV instances."""
+        dt = self._PREC_DTYPE[prec]
+        x = np.ascontiguousarray(x, dt)
+        y = np.empty(self.N, dt)
+        wv = None if w is None else np.ascontiguousarray(w, dt)
+        dots = (C.c_double * 2)()
+        self._check(lib.bda_spmv_perm_ex(self._h, prec, _p(x), _p(y), int(transpose), _p(wv), red, dots),
+                    "bda_spmv_perm_ex")
+        return (y, (dots[0], dots[1])) if red else y
+
+    def precond_apply_perm_ex(self, p, prec: int = 0, transpose: bool = False) -> np.ndarray:
+        dt = self._PREC_DTYPE[prec]
+        p = np.ascontiguousarray(p, dt)
+        z = np.empty(self.N, dt)
+        self._check(lib.bda_precond_apply_perm_ex(self._h, prec, _p(p), _p(z), int(transpose)),
+                    "bda_precond_apply_perm_ex")
         return z
 
     def factors(self):

@@ -137,6 +137,12 @@ int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx,
 /* New values on the uploaded pattern (Newton step); no reallocation. */
 int bda_update_values(bda_handle h, const double *values);
 int bda_update_values_dev(bda_handle h, const double *d_values);
+/* A counter that changes whenever the handle's matrix values or factors
+ * change (upload, value update, factorisation), so that a caller holding
+ * results of an earlier factorisation (an autograd backward pass) can tell
+ * whether the handle still holds that system.  No reference counterpart
+ * (the reference's buffers are rewritten by its caller only). */
+int bda_values_generation(bda_handle h, uint64_t *gen);
 
 /* Greedy colouring + colour-ordered permutation (host).  Counterpart of the
  * colour sizes / P-index arrays the reference consumes
@@ -217,6 +223,24 @@ int bda_grad_values_dev(bda_handle h, const double *d_lambda, const double *d_x,
 int bda_spmv_perm(bda_handle h, const double *x, double *y, int32_t transpose);
 /* z = M^{-1} p with M = LU (transpose != 0: z = M^{-T} p). */
 int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t transpose);
+/* The same in each kernel form the solves run, so that every form's kernels
+ * can be checked on their own.  prec selects the form and the vector type:
+ *   0  FP64 values, double vectors (bda_solve, the adjoint);
+ *   1  FP32 values, float vectors (bda_solve_mixed with vec32 != 0): FP32
+ *      products, FP64 sums, D^-1 in FP64, each output rounded to FP32;
+ *   2  FP32 values, double vectors (bda_solve_mixed with vec32 = 0): each x
+ *      entry rounded to FP32 before its FP32 product, FP64 sums.
+ * red selects the SpMV instance with the reductions the solve fuses into it:
+ * 0 none; 1 the v = A phat launch, dots[0] = y.w; 3 the t = A shat launch,
+ * dots[0] = y.w and dots[1] = y.y (w: a vector of the same type as x; forward
+ * only).  The reductions are summed in the solve's fixed order.  The
+ * transposed FP32 operators exist for this access only (the adjoint runs in
+ * FP64).  Counterparts: the SpMV mode and the ILU0 sweeps of the reference
+ * kernel (rtl/bicgstab_solver_2r-3r3w/spmvp.vhd:240-383, ilu0.vhd:385-416)
+ * with dot1/dot2 (vector_ops.vhd:301-309). */
+int bda_spmv_perm_ex(bda_handle h, int32_t prec, const void *x, void *y, int32_t transpose, const void *w,
+                     int32_t red, double dots[2]);
+int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z, int32_t transpose);
 /* Factors in the colour-ordered numbering: L and U in the strictly-lower /
  * strictly-upper block pattern of the permuted matrix (block counts from
  * bda_factor_sizes), dinv[n*bs*bs] the inverted diagonal blocks. */

@@ -112,6 +112,8 @@ def lib():
             "orc_factor": [_vp], "orc_factor_sizes": [_vp, C.POINTER(_i32), C.POINTER(_i32)],
             "orc_get_factors": [_vp] * 8, "orc_spmv_perm": [_vp, _vp, _vp, _i32],
             "orc_precond_perm": [_vp, _vp, _vp, _i32],
+            "orc_spmv_perm_ex": [_vp, _i32, _vp, _vp, _i32],
+            "orc_precond_perm_ex": [_vp, _i32, _vp, _vp, _i32],
             "orc_solve": [_vp, _vp, _vp, _f64, _i32, _i32, C.POINTER(OrcResult)],
             "orc_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, _i32, _f64, C.POINTER(OrcResult)],
             "orc_adjoint_solve": [_vp, _vp, _vp, _f64, _i32, C.POINTER(OrcResult)],
@@ -212,6 +214,26 @@ class OracleSystem:
         z = np.empty(self.N)
         if lib().orc_precond_perm(self._h, _p(p), _p(z), int(transpose)) != 0:
             raise RuntimeError("orc_precond_perm failed")
+        return z
+
+    _PREC_DTYPE = {0: np.float64, 1: np.float32, 2: np.float64}
+
+    def spmv_perm_ex(self, x, prec=0, transpose=False):
+        """y = A x in kernel form prec (0 FP64; 1 FP32 values and vectors;
+        2 FP32 values, FP64 vectors)."""
+        dt = self._PREC_DTYPE[prec]
+        x = np.ascontiguousarray(x, dt)
+        y = np.empty(self.N, dt)
+        if lib().orc_spmv_perm_ex(self._h, prec, _p(x), _p(y), int(transpose)) != 0:
+            raise RuntimeError("orc_spmv_perm_ex failed")
+        return y
+
+    def precond_perm_ex(self, p, prec=0, transpose=False):
+        dt = self._PREC_DTYPE[prec]
+        p = np.ascontiguousarray(p, dt)
+        z = np.empty(self.N, dt)
+        if lib().orc_precond_perm_ex(self._h, prec, _p(p), _p(z), int(transpose)) != 0:
+            raise RuntimeError("orc_precond_perm_ex failed")
         return z
 
     def solve(self, b, x0=None, tol=1e-6, maxit=1000, abs_tol=False):

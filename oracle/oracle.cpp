@@ -357,6 +357,8 @@ struct orc_system {
     bool adj_ready = false;
     BMat<float> A32, L32, U32;
     bool mixed_ready = false;
+    BMat<float> At32, Ut32, Lt32;      // FP32 transposes (component checks)
+    bool adj32_ready = false;
     std::vector<double> last_hist;     // norm history of the last orc_solve / orc_adjoint_solve
 };
 
@@ -474,7 +476,7 @@ int orc_update_values(orc_handle h, const double *values) {
     std::memcpy(h->orig.val.data(), values, nnzb * bb * 8);
     for (int64_t q = 0; q < nnzb; ++q)
         std::memcpy(&h->A.val[q * bb], &values[(int64_t)h->srcblk[q] * bb], bb * 8);
-    h->factored = h->adj_ready = h->mixed_ready = false;
+    h->factored = h->adj_ready = h->mixed_ready = h->adj32_ready = false;
     return 0;
 }
 
@@ -540,7 +542,7 @@ int orc_factor(orc_handle h) {
         for (int r = 0; r < bs; ++r)
             for (int c = 0; c < bs; ++c) h->dinvT[(size_t)i * bb + c * bs + r] = h->dinv[(size_t)i * bb + r * bs + c];
     h->factored = true;
-    h->adj_ready = h->mixed_ready = false;
+    h->adj_ready = h->mixed_ready = h->adj32_ready = false;
     return 0;
 }
 
@@ -572,6 +574,47 @@ static void ensure_adj(orc_handle h) {
     h->adj_ready = true;
 }
 
+static void ensure_mixed(orc_handle h) {
+    if (h->mixed_ready) return;
+    h->A32 = convert<float>(h->A);
+    h->L32 = convert<float>(h->L);
+    h->U32 = convert<float>(h->U);
+    h->mixed_ready = true;
+}
+
+static void ensure_adj32(orc_handle h) {
+    if (h->adj32_ready) return;
+    ensure_adj(h);
+    h->At32 = convert<float>(h->At);
+    h->Ut32 = convert<float>(h->Ut);
+    h->Lt32 = convert<float>(h->Lt);
+    h->adj32_ready = true;
+}
+
+// The precision form's operators: FP32 values, vectors of type TX
+extern "C++" {
+template <typename TX>
+static void spmv32(orc_handle h, const TX *x, TX *y, bool transpose) {
+    ensure_mixed(h);
+    if (transpose) ensure_adj32(h);
+    spmv(transpose ? h->At32 : h->A32, x, y);
+}
+
+template <typename TX>
+static void precond32(orc_handle h, const TX *p, TX *z, bool transpose) {
+    ensure_mixed(h);
+    std::vector<TX> y((size_t)h->n * h->bs);
+    if (transpose) {
+        ensure_adj32(h);
+        sweep(h->Ut32, h->cptr, true, h->dinvT.data(), p, y.data());
+        sweep(h->Lt32, h->cptr, false, (const double *)nullptr, y.data(), z);
+    } else {
+        sweep(h->L32, h->cptr, true, (const double *)nullptr, p, y.data());
+        sweep(h->U32, h->cptr, false, h->dinv.data(), y.data(), z);
+    }
+}
+}  // extern "C++"
+
 int orc_spmv_perm(orc_handle h, const double *x, double *y, int32_t transpose) {
     if (transpose) {
         if (!h->factored) return 1;
@@ -595,6 +638,28 @@ int orc_precond_perm(orc_handle h, const double *p, double *z, int32_t transpose
         sweep(h->U, h->cptr, false, h->dinv.data(), y.data(), z);
     }
     return 0;
+}
+
+// prec 0: FP64 (double vectors); 1: FP32 values and float vectors; 2: FP32
+// values and double vectors (each x entry rounded to FP32 in the product)
+int orc_spmv_perm_ex(orc_handle h, int32_t prec, const void *x, void *y, int32_t transpose) {
+    if (!h->factored && (prec != 0 || transpose)) return 1;
+    switch (prec) {
+    case 0: return orc_spmv_perm(h, (const double *)x, (double *)y, transpose);
+    case 1: spmv32<float>(h, (const float *)x, (float *)y, transpose != 0); return 0;
+    case 2: spmv32<double>(h, (const double *)x, (double *)y, transpose != 0); return 0;
+    default: return 2;
+    }
+}
+
+int orc_precond_perm_ex(orc_handle h, int32_t prec, const void *p, void *z, int32_t transpose) {
+    if (!h->factored) return 1;
+    switch (prec) {
+    case 0: return orc_precond_perm(h, (const double *)p, (double *)z, transpose);
+    case 1: precond32<float>(h, (const float *)p, (float *)z, transpose != 0); return 0;
+    case 2: precond32<double>(h, (const double *)p, (double *)z, transpose != 0); return 0;
+    default: return 2;
+    }
 }
 
 static void permute_in(orc_handle h, const double *in, double *out) {
@@ -666,12 +731,7 @@ extern "C" {
 int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
                     int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res) {
     if (!h->factored) return 1;
-    if (!h->mixed_ready) {
-        h->A32 = convert<float>(h->A);
-        h->L32 = convert<float>(h->L);
-        h->U32 = convert<float>(h->U);
-        h->mixed_ready = true;
-    }
+    ensure_mixed(h);
     const int64_t N = (int64_t)h->n * h->bs;
     std::vector<double> bp(N), xo(N), ro(N), v(N);
     permute_in(h, b, bp.data());
@@ -713,6 +773,9 @@ int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_
         if (st.breakdown) r.breakdown = 1;
         if (st.nan) r.nan_detected = 1;
         if (st.no_results) break;
+        // a breakdown or NaN before the first half-iteration completed left x
+        // unchanged: the next outer step would repeat the same inner run
+        if ((st.breakdown || st.nan) && st.h == 0 && st.hist.size() == 1) break;
     }
     r.rel_residual = nrm0 > 0 ? r.final_norm / nrm0 : 0.0;
     r.half_iterations = r.inner_half_iterations;

@@ -50,6 +50,10 @@ int orc_get_factors(orc_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *
 /* colour-ordered numbering */
 int orc_spmv_perm(orc_handle h, const double *x, double *y, int32_t transpose);
 int orc_precond_perm(orc_handle h, const double *p, double *z, int32_t transpose);
+/* prec 0: FP64 (double vectors); 1: FP32 values, float vectors; 2: FP32
+ * values, double vectors -- the three kernel forms of the solves */
+int orc_spmv_perm_ex(orc_handle h, int32_t prec, const void *x, void *y, int32_t transpose);
+int orc_precond_perm_ex(orc_handle h, int32_t prec, const void *p, void *z, int32_t transpose);
 /* original numbering */
 int orc_solve(orc_handle h, const double *b, double *x, double tol, int32_t maxit, int32_t abs_tol,
               orc_result *res);

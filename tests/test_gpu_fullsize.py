@@ -45,6 +45,17 @@ def test_full_components_bit_exact(big):
     assert h(gl) == h(ol) and h(gu) == h(ou) and h(gd) == h(od)
 
 
+def test_full_fp32_components_bit_exact(big):
+    """The precision form's kernels at full size: FP32 SpMV and both sweeps
+    (float vectors, as in the timed solve) equal the oracle's FP32 path."""
+    s, g, o = big
+    x = np.random.default_rng(8).uniform(-1, 1, s.N).astype(np.float32)
+    assert h(g.spmv_perm_ex(x, 1)) == h(o.spmv_perm_ex(x, 1))
+    assert h(g.precond_apply_perm_ex(x, 1)) == h(o.precond_perm_ex(x, 1))
+    y, d = g.spmv_perm_ex(x, 1, w=x, red=3)
+    assert h(y) == h(o.spmv_perm_ex(x, 1))
+
+
 def test_full_solve_parity(big):
     s, g, o = big
     xg, rg = g.solve(s.b, tol=1e-6, maxit=1000)

@@ -142,7 +142,29 @@ def test_level_scheduling_solve():
         np.testing.assert_array_equal(a, b)
     xg, rg = g.solve(s.b, tol=1e-8, maxit=200)
     xo, ro = o.solve(s.b, tol=1e-8, maxit=200)
-    assert rg.converged and abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    assert rg.converged and ro["converged"] and abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    if rg.half_iterations == ro["half_iterations"]:
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-8
+    # the level-set sweeps bit for bit, as for the colouring
+    p = np.random.default_rng(9).uniform(-1, 1, s.N)
+    np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
+    np.testing.assert_array_equal(g.spmv_perm(p), o.spmv_perm(p))
+    g.close()
+
+
+def test_mixed_inner_breakdown_ends_refinement():
+    """A right-hand side finite in FP64 but beyond the FP32 range: the first
+    inner run meets Inf/NaN before its first half-iteration and changes
+    nothing, so the refinement stops after that step (it would otherwise
+    repeat the same inner run max_outer times), on the GPU as in the oracle."""
+    s, g, o = _pair(scaled(GRID_32x32x16, 6, 5, 4))
+    b = s.b * 1e39
+    xg, rg = g.solve_mixed(b, tol=1e-6, max_outer=20, vec32=True)
+    xo, ro = o.solve_mixed(b, tol=1e-6, max_outer=20, vec32=True)
+    assert rg.nan_detected and ro["nan_detected"] and not rg.converged and not ro["converged"]
+    assert rg.outer_iterations == ro["outer_iterations"] == 1
+    assert rg.inner_half_iterations == ro["inner_half_iterations"] == 0
+    assert np.all(xg == 0) and np.all(xo == 0)
     g.close()
 
 
@@ -258,8 +280,12 @@ def test_edge_watchdog_abort():
     g.set_options(timeout_s=1e-9)
     x, r = g.solve(s.b, tol=1e-14, maxit=1000)
     assert r.aborted and not r.converged
+    # the deadline is checked on the device: the run stops at its first
+    # scalar step, and what the host had queued does no work
+    assert r.half_iterations == 0 and np.all(x == 0)
     xm, rm = g.solve_mixed(s.b, tol=1e-14)
     assert rm.aborted and not rm.converged
+    assert rm.outer_iterations == 0 and np.all(xm == 0)
     g.set_options(timeout_s=0.0)
     x, r = g.solve(s.b, tol=1e-8, maxit=300)
     xo, ro = o.solve(s.b, tol=1e-8, maxit=300)

@@ -78,12 +78,20 @@ def test_precond_bit_exact_tall_tiles(pair):
     p = np.random.default_rng(5).uniform(-1, 1, len(b))
     np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
     np.testing.assert_array_equal(g.precond_apply_perm(p, transpose=True), o.precond_perm(p, transpose=True))
+    # the precision form's instances of the same launches (FP32 values; float
+    # and double vectors)
+    for prec, dt in ((1, np.float32), (2, np.float64)):
+        for tr in (False, True):
+            np.testing.assert_array_equal(g.precond_apply_perm_ex(p.astype(dt), prec, transpose=tr),
+                                          o.precond_perm_ex(p.astype(dt), prec, transpose=tr))
 
 
 def test_spmv_bit_exact(pair):
     bs, b, g, o = pair
     x = np.random.default_rng(6).uniform(-1, 1, len(b))
     np.testing.assert_array_equal(g.spmv_perm(x), o.spmv_perm(x))
+    for prec, dt in ((1, np.float32), (2, np.float64)):
+        np.testing.assert_array_equal(g.spmv_perm_ex(x.astype(dt), prec), o.spmv_perm_ex(x.astype(dt), prec))
 
 
 def test_solve_parity(pair):
@@ -145,6 +153,10 @@ def test_precond_bit_exact_tail_limit(bs, nlast, fan):
         p = np.random.default_rng(7).uniform(-1, 1, (len(rowptr) - 1) * bs)
         np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
         np.testing.assert_array_equal(g.precond_apply_perm(p, transpose=True), o.precond_perm(p, transpose=True))
+        for prec, dt in ((1, np.float32), (2, np.float64)):
+            for tr in (False, True):
+                np.testing.assert_array_equal(g.precond_apply_perm_ex(p.astype(dt), prec, transpose=tr),
+                                              o.precond_perm_ex(p.astype(dt), prec, transpose=tr))
         b = np.random.default_rng(8).uniform(-1, 1, len(p))
         xg, rg = g.solve(b, tol=1e-10, maxit=200)
         xo, ro = o.solve(b, tol=1e-10, maxit=200)

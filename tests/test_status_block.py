@@ -104,3 +104,77 @@ def test_query_roundtrip():
     assert s16[0].value == 128 * 3 and max_matrix.value == 6003904 * 9
     assert (b[0].value, b[1].value, s16[1].value, u8[0].value) == (False, True, 512, 64)
     assert (u8[4].value, u8[5].value) == (1, 1)
+
+
+def decode_query(line):
+    buf = (C.c_ulong * 8)(*[int(v) for v in line])
+    u = [C.c_uint() for _ in range(4)]
+    s16 = [C.c_ushort() for _ in range(4)]
+    b = [C.c_bool() for _ in range(2)]
+    u8 = [C.c_ubyte() for _ in range(6)]
+    max_matrix = C.c_uint()
+    rc = dec[1](C.c_bool(False), buf, C.c_uint(1), C.c_uint(8), C.byref(u[0]), C.byref(u[1]), C.byref(u[2]),
+                C.byref(u[3]), C.byref(s16[0]), C.byref(max_matrix), C.byref(b[0]), C.byref(b[1]),
+                C.byref(s16[1]), C.byref(u8[0]), C.byref(u8[1]), C.byref(u8[2]), C.byref(u8[3]), C.byref(u8[4]),
+                C.byref(u8[5]), C.byref(s16[2]), C.byref(s16[3]))
+    return dict(rc=rc, x_elems=u[0].value, max_rows=u[1].value, max_cols=u[2].value, max_colours=u[3].value,
+                nnz_per_row=s16[0].value, max_matrix=max_matrix.value, uram=b[0].value, ilu0_out=b[1].value,
+                width=s16[1].value, mults=u8[0].value, rd_ports=u8[4].value, wr_ports=u8[5].value)
+
+
+@pytest.mark.gpu
+def test_live_query_decodes():
+    """bda_query on a live handle (fpga_kernel_query, fpga_functions_bicgstab.cpp:1332-1452), its limits
+    line decoded by the reference's decode_debuginfo_bicgstab_query (bicgstab_utils.cpp:282-369)."""
+    s = fa.generate(fa.configs.scaled(fa.configs.GRID_32x32x16, 10, 8, 6))
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    try:
+        lim = g.query()
+        print("live limits", {k: getattr(lim, k) for k, _ in lim._fields_}, flush=True)
+        assert lim.max_block_size == 4 and lim.max_iterations == 32767
+        assert lim.wavefront_size == 64 and lim.compute_units >= 200
+        assert lim.device_memory_bytes > (200 << 30)
+        assert lim.max_blocks_per_row == 168 and lim.rows_per_tile == 64
+        line = (C.c_uint64 * 8)()
+        assert lib.bda_encode_query(C.byref(lim), s.n, s.nnzb, s.bs, line) == 0
+        d = decode_query(list(line))
+        assert d["rc"] == 0
+        assert (d["x_elems"], d["max_rows"], d["max_cols"], d["max_colours"]) == (s.N, s.N, s.N, s.n)
+        assert d["nnz_per_row"] == 168 * 3 and d["max_matrix"] == s.nnzb * 9
+        assert (d["uram"], d["ilu0_out"], d["width"], d["mults"]) == (False, True, 512, 64)
+        assert (d["rd_ports"], d["wr_ports"]) == (1, 1)
+    finally:
+        g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tol,maxit", [(1e-2, 200), (1e-8, 200), (0.0, 3), (0.0, 4)])
+def test_gpu_solve_status_decodes(tol, maxit):
+    """A real GPU solve's result encoded as the kernel's status line and read
+    back by decode_debuginfo_bicgstab (bicgstab_utils.cpp:90-280): half
+    iterations, the four norms, the newest-norm index and the flags."""
+    s = fa.generate(fa.configs.scaled(fa.configs.GRID_32x32x16, 10, 8, 6))
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    try:
+        g.upload_system(s)
+        g.analyse()
+        g.factor()
+        r = BdaResult()
+        x = np.zeros(s.N)
+        assert lib.bda_solve(C.c_void_p(g.handle), s.b.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                             tol, maxit, C.byref(r)) == 0
+        d = decode_status(encode(r))
+        assert d["rc"] == 0 and not d["signature_bad"] and not d["aborted"] and not d["noresults"]
+        assert d["iters"] == r.half_iterations and d["last"] == r.last_norm_idx
+        assert d["norms"] == list(r.norms) and d["norms"][d["last"]] == r.final_norm
+        assert abs(d["cycles"] - r.solve_ms * 1e-3 * 250e6) <= 0.5  # cycles of the 250 MHz kernel clock
+        # no results: a zero right-hand side and start vector
+        z = np.zeros(s.N)
+        x = np.zeros(s.N)
+        assert lib.bda_solve(C.c_void_p(g.handle), z.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                             tol, maxit, C.byref(r)) == 0
+        assert r.no_results
+        d = decode_status(encode(r))
+        assert d["noresults"] and d["iters"] == 0
+    finally:
+        g.close()
