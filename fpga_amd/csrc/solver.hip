@@ -462,7 +462,7 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     if (!use_state) a.ra.st = h.st.p + kIdleState;
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
-    if (nred > 0 && a.ntiles > h.partial_cap) throw std::runtime_error("partials buffer too small");
+    if (nred > 0 && a.ntiles > a.ra.cap) throw std::runtime_error("partials buffer too small");
     if (prof_kind >= 0) prof_begin(h, prof_kind);
     int nparts = 0;
     if constexpr (MODE != SM_SPMV && RED == SR_NONE) {
@@ -1107,31 +1107,70 @@ enum Prec : int { PREC_F64 = 0, PREC_F32 = 1, PREC_F32_X64 = 2 };
 // (v = A phat with v.rt; t = A shat with t.s and t.t), summed in the
 // solve's fixed order into dots[].  Vectors are device buffers of the
 // handle in the colour-ordered numbering.
+// Where the fused reductions of a component SpMV go: the partial sums
+// (2 x cap doubles) and the scalar state that adds them up.  nullptr: the
+// handle's own (partials and state slot 1).
+struct RedScratch {
+    double *partials;
+    int32_t cap;
+    SolverState *st;
+};
+
 template <int BS, typename TV, typename TX>
-void spmv_comp(H &h, const TX *x, TX *y, int transpose, const TX *w, int redm, double *dots) {
+void spmv_comp(H &h, const TX *x, TX *y, int transpose, const TX *w, int redm, double *dots,
+               const RedScratch *sc = nullptr) {
     const DevMat &m = transpose ? h.At : h.A;
     const TV *val;
     if constexpr (std::is_same<TV, float>::value) val = transpose ? h.At32.p : h.A32.p;
     else val = transpose ? h.At64.p : h.A64.p;
+    SolverState *st = sc ? sc->st : h.st.p + 1;
     if (redm == 0) {
         launch_stream<BS, SM_SPMV, SR_NONE, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, nullptr,
                                                     red(h, 1, FIN_NONE), false);
     } else {
-        init_state(h, 1, 0.0, 0, false);
+        RedArgs ra = red(h, 1, FIN_STORE);
+        if (sc) {
+            // a fresh state (no host flags: FIN_STORE raises none), written
+            // by a kernel on the stream
+            SolverState v;
+            std::memset(&v, 0, sizeof(v));
+            v.norm_slot = 1;
+            hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, h.stream, sc->st, v);
+            check_launch();
+            ra.partials = sc->partials;
+            ra.cap = sc->cap;
+            ra.st = sc->st;
+        } else {
+            init_state(h, 1, 0.0, 0, false);
+        }
         if (redm == 1)
-            launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w,
-                                                      red(h, 1, FIN_STORE), true);
+            launch_stream<BS, SM_SPMV, SR_W0, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w, ra, true);
         else
-            launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w,
-                                                        red(h, 1, FIN_STORE), true);
+            launch_stream<BS, SM_SPMV, SR_BOTH, TV, TX>(h, m, val, -1, x, nullptr, y, nullptr, w, ra, true);
     }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     if (redm != 0 && dots) {
         SolverState s;
-        HIP_CHECK(hipMemcpy(&s, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost));
+        HIP_CHECK(hipMemcpy(&s, st, sizeof(SolverState), hipMemcpyDeviceToHost));
         dots[0] = s.tot[0];
         dots[1] = redm == 3 ? s.tot[1] : 0.0;
     }
+}
+
+template <int BS>
+void spmv_comp_prec(H &h, int prec, const void *x, void *y, int transpose, const void *w, int redm, double *dots,
+                    const RedScratch *sc = nullptr) {
+    if (redm != 0 && redm != 1 && redm != 3) throw std::runtime_error("red must be 0, 1 or 3");
+    if (redm != 0 && transpose) throw std::runtime_error("fused reductions: forward operator only");
+    if (redm != 0 && !w) throw std::runtime_error("the fused reductions need w");
+    if (prec == PREC_F32)
+        spmv_comp<BS, float, float>(h, (const float *)x, (float *)y, transpose, (const float *)w, redm, dots, sc);
+    else if (prec == PREC_F32_X64)
+        spmv_comp<BS, float, double>(h, (const double *)x, (double *)y, transpose, (const double *)w, redm, dots,
+                                     sc);
+    else
+        spmv_comp<BS, double, double>(h, (const double *)x, (double *)y, transpose, (const double *)w, redm, dots,
+                                      sc);
 }
 
 template <int BS>
@@ -1152,20 +1191,18 @@ void comp_prepare(H &h, int prec, int transpose) {
 template <int BS>
 void spmv_perm(H &h, int prec, const void *x, void *y, int transpose, const void *w, int redm, double *dots) {
     comp_prepare<BS>(h, prec, transpose);
-    if (redm != 0 && redm != 1 && redm != 3) throw std::runtime_error("red must be 0, 1 or 3");
-    if (redm != 0 && transpose) throw std::runtime_error("fused reductions: forward operator only");
-    if (redm != 0 && !w) throw std::runtime_error("the fused reductions need w");
     const size_t es = prec == PREC_F32 ? 4 : 8;
     HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * es, hipMemcpyHostToDevice));
     if (w) HIP_CHECK(hipMemcpy(h.vw.p, w, h.N() * es, hipMemcpyHostToDevice));
-    if (prec == PREC_F32)
-        spmv_comp<BS, float, float>(h, (const float *)h.vin.p, (float *)h.vout.p, transpose, (const float *)h.vw.p,
-                                    redm, dots);
-    else if (prec == PREC_F32_X64)
-        spmv_comp<BS, float, double>(h, h.vin.p, h.vout.p, transpose, h.vw.p, redm, dots);
-    else
-        spmv_comp<BS, double, double>(h, h.vin.p, h.vout.p, transpose, h.vw.p, redm, dots);
+    spmv_comp_prec<BS>(h, prec, h.vin.p, h.vout.p, transpose, w ? h.vw.p : nullptr, redm, dots);
     HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * es, hipMemcpyDeviceToHost));
+}
+
+// The same on the caller's device vectors.
+template <int BS>
+void spmv_dev(H &h, int prec, const void *d_x, void *d_y, int transpose, const void *d_w, int redm, double *dots) {
+    comp_prepare<BS>(h, prec, transpose);
+    spmv_comp_prec<BS>(h, prec, d_x, d_y, transpose, d_w, redm, dots);
 }
 
 template <int BS>
@@ -1547,6 +1584,14 @@ int bda_spmv_perm_ex(bda_handle h, int32_t prec, const void *x, void *y, int32_t
     return guarded(h, [&] {
         if (!x || !y) throw std::runtime_error("null vector");
         BS_CALL(h, spmv_perm, *h, prec, x, y, transpose, w, red, dots);
+    });
+}
+
+int bda_spmv_dev(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t transpose, const void *d_w,
+                 int32_t red, double dots[2]) {
+    return guarded(h, [&] {
+        if (!d_x || !d_y) throw std::runtime_error("null vector");
+        BS_CALL(h, spmv_dev, *h, prec, d_x, d_y, transpose, d_w, red, dots);
     });
 }
 

@@ -332,6 +332,14 @@ class BicgstabSolver:
                     "bda_spmv_perm_ex")
         return (y, (dots[0], dots[1])) if red else y
 
+    def spmv_dev(self, x_ptr: int, y_ptr: int, prec: int = 0, transpose: bool = False, w_ptr: int = 0,
+                 red: int = 0):
+        """y = A x on device vectors (colour-ordered); returns the fused dots."""
+        dots = (C.c_double * 2)()
+        self._check(lib.bda_spmv_dev(self._h, prec, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(transpose),
+                                     C.c_void_p(w_ptr) if w_ptr else None, red, dots), "bda_spmv_dev")
+        return dots[0], dots[1]
+
     def precond_apply_perm_ex(self, p, prec: int = 0, transpose: bool = False) -> np.ndarray:
         dt = self._PREC_DTYPE[prec]
         p = np.ascontiguousarray(p, dt)

@@ -241,6 +241,16 @@ int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t tra
 int bda_spmv_perm_ex(bda_handle h, int32_t prec, const void *x, void *y, int32_t transpose, const void *w,
                      int32_t red, double dots[2]);
 int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z, int32_t transpose);
+/* y = A x (transpose != 0: A^T x) on the caller's device vectors (float for
+ * prec 1, double otherwise) in the colour-ordered numbering, with the fused
+ * reductions of bda_spmv_perm_ex; for callers that drive their own Krylov
+ * loop on the handle's matrix.  Counterpart of the reference kernel's SpMV
+ * mode (rtl/bicgstab_solver_2r-3r3w/spmvp.vhd:240-383, sparstition.vhd:
+ * 655-878) with dot1/dot2 (vector_ops.vhd:301-309).  Its partial sums and
+ * scalar state are the handle's; the first FP32 or transposed call makes
+ * the handle's FP32 or transposed copies of the matrix. */
+int bda_spmv_dev(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t transpose, const void *d_w,
+                 int32_t red, double dots[2]);
 /* Factors in the colour-ordered numbering: L and U in the strictly-lower /
  * strictly-upper block pattern of the permuted matrix (block counts from
  * bda_factor_sizes), dinv[n*bs*bs] the inverted diagonal blocks. */

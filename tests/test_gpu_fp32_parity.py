@@ -124,3 +124,24 @@ def test_mixed_fixed_iterations(system):
             # most ||A (xg - xo)|| plus the rounding of the two residuals
             dr = np.linalg.norm(A @ (xg - xo))
             assert abs(rg.final_norm - ro["final_norm"]) <= dr + 1e-12 * ro["norms"][0]
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_spmv_dev_on_caller_vectors(system, prec):
+    """bda_spmv_dev on torch device tensors equals the oracle's product and
+    the host-array access, including the fused sums."""
+    import torch
+    _, s, g, o = system
+    rng = np.random.default_rng(24)
+    x = rng.uniform(-1, 1, s.N).astype(DT[prec])
+    w = rng.uniform(-1, 1, s.N).astype(DT[prec])
+    dev = torch.device("cuda:0")
+    xd, wd = torch.from_numpy(x).to(dev), torch.from_numpy(w).to(dev)
+    yd = torch.empty_like(xd)
+    torch.cuda.synchronize()
+    d3 = g.spmv_dev(xd.data_ptr(), yd.data_ptr(), prec, w_ptr=wd.data_ptr(), red=3)
+    np.testing.assert_array_equal(yd.cpu().numpy(), o.spmv_perm_ex(x, prec))
+    _, dh = g.spmv_perm_ex(x, prec, w=w, red=3)
+    assert d3 == dh
+    g.spmv_dev(xd.data_ptr(), yd.data_ptr(), prec, transpose=True)
+    np.testing.assert_array_equal(yd.cpu().numpy(), o.spmv_perm_ex(x, prec, transpose=True))
