@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -325,6 +326,40 @@ struct bda_handle_s {
     size_t N() const { return (size_t)n * bs; }
     size_t Npad() const { return (size_t)n * bs + bda::kVecPad; }
 };
+
+// Scratch of bda_spmv_dev_noalloc: the scalar state and the reduction
+// partial sums of its SpMV, in the library's device code rather than in
+// memory the library allocates.  Every GPU the library runs on holds its own
+// copy at its own address until the process ends; every call on a GPU uses
+// that GPU's copy, so calls that overlap in time on one GPU race.  On the
+// host the name is only a stand-in: host code uses the device address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[size_t(1) << 20];
+
+namespace bda {
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = sizeof(scratch);
+constexpr size_t kNoAllocStateBytes = 512;  // the SolverState, padded
+static_assert(sizeof(SolverState) <= kNoAllocStateBytes, "state slot of the scratch array");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current (from bda_create).
+void noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+        throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+}
+
+}  // namespace
+}  // namespace bda
 
 namespace bda {
 namespace {
@@ -1116,6 +1151,22 @@ struct RedScratch {
     SolverState *st;
 };
 
+// The scratch array of a device as reduction space: the state in its first
+// 512 bytes, then 2 x cap partial sums.
+RedScratch noalloc_scratch(int device) {
+    void *p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device >= 0 && device < kMaxDevices) p = g_scratch_ptr[device];
+    }
+    if (!p) throw std::runtime_error("no scratch address for this device (bda_create sets it up)");
+    RedScratch sc;
+    sc.st = reinterpret_cast<SolverState *>(p);
+    sc.partials = reinterpret_cast<double *>(static_cast<unsigned char *>(p) + kNoAllocStateBytes);
+    sc.cap = (int32_t)((kNoAllocScratchBytes - kNoAllocStateBytes) / (2 * sizeof(double)));
+    return sc;
+}
+
 template <int BS, typename TV, typename TX>
 void spmv_comp(H &h, const TX *x, TX *y, int transpose, const TX *w, int redm, double *dots,
                const RedScratch *sc = nullptr) {
@@ -1203,6 +1254,39 @@ template <int BS>
 void spmv_dev(H &h, int prec, const void *d_x, void *d_y, int transpose, const void *d_w, int redm, double *dots) {
     comp_prepare<BS>(h, prec, transpose);
     spmv_comp_prec<BS>(h, prec, d_x, d_y, transpose, d_w, redm, dots);
+}
+
+// bda_spmv_dev without any device allocation: the handle's copies of the
+// matrix must already exist and be current, and the fused reductions use
+// the per-device scratch array instead of the handle's buffers.
+template <int BS>
+void spmv_dev_noalloc(H &h, int prec, const void *d_x, void *d_y, int transpose, const void *d_w, int redm,
+                      double *dots) {
+    require_analysed(h);
+    if (!h.have_values) throw std::runtime_error("no matrix values uploaded");
+    if (prec != PREC_F64 && prec != PREC_F32 && prec != PREC_F32_X64)
+        throw std::runtime_error("precision must be 0 (FP64), 1 (FP32 vectors) or 2 (FP64 vectors)");
+    if (prec != PREC_F64) {
+        require_factored(h);
+        if (h.mixed_gen != h.factor_gen)
+            throw std::runtime_error("the FP32 copies of the matrix are not current: call bda_spmv_dev or a "
+                                     "precision-form solve once after bda_factor");
+    }
+    if (transpose) {
+        require_factored(h);
+        if (h.adj_gen != h.factor_gen || (prec != PREC_F64 && h.adj32_gen != h.factor_gen))
+            throw std::runtime_error("the transposed copies of the matrix are not current: call bda_spmv_dev "
+                                     "with transpose once after bda_factor");
+    }
+    const DevMat &m = transpose ? h.At : h.A;
+    const int ntiles = wide_tiles<SM_SPMV>() ? m.ntiles_wide : m.ntiles;
+    const RedScratch sc = noalloc_scratch(h.device);
+    if (redm != 0 && ntiles > sc.cap)
+        throw std::runtime_error("this system's SpMV writes " + std::to_string(ntiles) +
+                                 " partial sums per reduction; the no-allocation entry point holds at most " +
+                                 std::to_string(sc.cap) + " (" + std::to_string(kNoAllocScratchBytes >> 10) +
+                                 " KiB of scratch)");
+    spmv_comp_prec<BS>(h, prec, d_x, d_y, transpose, d_w, redm, dots, &sc);
 }
 
 template <int BS>
@@ -1316,6 +1400,7 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         HIP_CHECK(hipGetDevice(&h->device));
         HIP_CHECK(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device));
         HIP_CHECK(hipDeviceGetAttribute(&h->wall_khz, hipDeviceAttributeWallClockRate, h->device));
+        noalloc_scratch_setup(h->device);
         if (h->wall_khz <= 0) throw std::runtime_error("device reports no wall clock rate");
         // a blocking stream: it orders itself after work the caller queued on
         // the legacy default stream (torch's default), so device inputs
@@ -1592,6 +1677,14 @@ int bda_spmv_dev(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t
     return guarded(h, [&] {
         if (!d_x || !d_y) throw std::runtime_error("null vector");
         BS_CALL(h, spmv_dev, *h, prec, d_x, d_y, transpose, d_w, red, dots);
+    });
+}
+
+int bda_spmv_dev_noalloc(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t transpose,
+                         const void *d_w, int32_t red, double dots[2]) {
+    return guarded(h, [&] {
+        if (!d_x || !d_y) throw std::runtime_error("null vector");
+        BS_CALL(h, spmv_dev_noalloc, *h, prec, d_x, d_y, transpose, d_w, red, dots);
     });
 }
 

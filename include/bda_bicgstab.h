@@ -251,6 +251,22 @@ int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z
  * the handle's FP32 or transposed copies of the matrix. */
 int bda_spmv_dev(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t transpose, const void *d_w,
                  int32_t red, double dots[2]);
+/* bda_spmv_dev without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or allocating routine: the
+ * fused reductions keep their partial sums and scalar state in a 1 MiB array
+ * of the library's device code (one per GPU, held until the process ends),
+ * and the handle's copies of the matrix must already exist and be current
+ * (prec 1/2: the FP32 copies, made by a bda_spmv_dev or precision-form call
+ * after the last bda_factor; transpose: the transposed copies, made by a
+ * transposed bda_spmv_dev or an adjoint solve), else it fails with a
+ * message.  A system whose SpMV writes more than 65,504 partial sums per
+ * reduction (its launch's tile count) is rejected when red != 0.  All calls
+ * on one GPU share its array: calls that overlap in time on one GPU (two
+ * threads, two streams) race.  Results equal bda_spmv_dev's bit for bit. */
+int bda_spmv_dev_noalloc(bda_handle h, int32_t prec, const void *d_x, void *d_y, int32_t transpose,
+                         const void *d_w, int32_t red, double dots[2]);
 /* Factors in the colour-ordered numbering: L and U in the strictly-lower /
  * strictly-upper block pattern of the permuted matrix (block counts from
  * bda_factor_sizes), dinv[n*bs*bs] the inverted diagonal blocks. */
