@@ -12,6 +12,18 @@ per solve; the FP64 time, iteration counts, the error of each against a
 1e-12 FP64 solve, the roofline of the dominant kernel (the precision form's
 FP32 SpMV, timed inside a solve) and the CPU oracle baseline are in the same
 JSON line.
+
+Beside the headline the line carries, measured in the same run:
+  trisolve   back-to-back preconditioner applications (all L and U sweep
+             launches), FP64 and FP32, against their algorithmic bytes;
+  adjoint    the 64x64x32 grid's adjoint backward (A^T lambda = w to 1e-10,
+             then the gradient -lambda x^T on the pattern);
+  grid256x256x64  the 4x4-block grid: FP64 forward, precision-form forward and
+             adjoint backward, each with its time and algorithmic GB/s;
+  cpu_baseline    the CPU oracle on the host cores: the 128x128x64 solve (all
+             threads, median of 3; one thread, one full solve) and the
+             32x32x16 solve of the CPU-runnable case (median of 5, all threads
+             and one thread, with the colouring's SHA-256).
 """
 from __future__ import annotations
 
@@ -43,8 +55,8 @@ def parse():
                     help="kernels: only repeated SpMV launches (FP64 and FP32 values) and a "
                          "reference copy of known size, for rocprofv3 counter runs")
     ap.add_argument("--reps", type=int, default=20)
-    ap.add_argument("--adjoint", action="store_true",
-                    help="also time the adjoint backward (A^T lambda = w to 1e-10, then -lambda x^T)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the adjoint (64x64x32) and 4x4 (256x256x64) blocks")
     return ap.parse_args()
 
 
@@ -82,7 +94,10 @@ def main():
     g.factor()
     torch.cuda.synchronize()
     t_fac = time.perf_counter() - t0
-    log(f"analyse {t_an:.3f}s factor {t_fac:.3f}s colours {g.num_colours}")
+    t0 = time.perf_counter()
+    g.prepare_mixed()     # the precision form's FP32 copies (paid again after every factorisation)
+    t_dem = time.perf_counter() - t0
+    log(f"analyse {t_an:.3f}s factor {t_fac:.3f}s fp32 copies {t_dem:.4f}s colours {g.num_colours}")
 
     b = torch.from_numpy(s.b).to(dev)
     x = torch.zeros(s.N, dtype=torch.float64, device=dev)
@@ -144,27 +159,6 @@ def main():
     spmv32_in = in_solve_avg(g.kernel_times(fa.KIND_SPMV))
     g.set_profiling(False)
 
-    adj = None
-    if args.adjoint:
-        w = torch.from_numpy(np.random.default_rng(0).standard_normal(s.N)).to(dev)
-        lam = torch.zeros_like(w)
-        grad = torch.empty((s.nnzb, s.bs, s.bs), dtype=torch.float64, device=dev)
-        g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, maxit)   # builds the transposed operators
-        times, ra = [], None
-        for _ in range(args.steps):
-            lam.zero_()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            ra = g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, maxit)
-            g.grad_values_dev(lam.data_ptr(), x_ref.data_ptr(), grad.data_ptr())
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - t0)
-        adj = {"time_s": float(np.mean(times)), "tol": 1e-10, "iterations": ra.iterations,
-               "converged": ra.converged, "rel_residual": ra.rel_residual}
-        adj_pending = True
-    else:
-        adj_pending = False
-
     spmv_bytes = g.algorithmic_bytes(fa.KIND_SPMV)
     spmv32_bytes = g.algorithmic_bytes(fa.KIND_SPMV, True)
     # roofline of the dominant kernel of the timed solve, the precision
@@ -187,42 +181,28 @@ def main():
     # algorithmic bytes per FP64 iteration: 2 SpMV + 2 (L + U) + 22 N vector passes
     it_bytes = 2 * spmv_bytes + 2 * (low_bytes + up_bytes) + 22 * s.N * 8
     solve_gbs = it_bytes * iters64 / t_fp64 / 1e9
-    if adj_pending:
-        # same per-iteration traffic (A^T, U^T, L^T have the blocks of A, U, L) plus the gradient pass
-        grad_bytes = s.nnzb * s.bs * s.bs * 8 + s.nnzb * 12 + 2 * s.N * 8
-        adj["GBs_algorithmic"] = (it_bytes * adj["iterations"] + grad_bytes) / adj["time_s"] / 1e9
     mixed_it_bytes = (2 * g.algorithmic_bytes(fa.KIND_SPMV, True)
                       + 2 * (g.algorithmic_bytes(fa.KIND_LOWER, True) + g.algorithmic_bytes(fa.KIND_UPPER, True))
                       + 22 * s.N * 4)
+    # one preconditioner application (L then U sweep) back to back, both forms
+    trisolve = {}
+    for fp32 in (False, True):
+        ms = g.time_precond(20, fp32)
+        by = g.algorithmic_bytes(fa.KIND_LOWER, fp32) + g.algorithmic_bytes(fa.KIND_UPPER, fp32)
+        trisolve["fp32" if fp32 else "fp64"] = {"avg_ms": ms, "algorithmic_bytes": by,
+                                                "GBs": by / (ms * 1e-3) / 1e9,
+                                                "frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    ncolours = g.num_colours
+    g.close()
+
+    extra_adj = extra_big = None
+    if not args.no_extra:
+        extra_adj = adjoint_block(fa, torch, dev, min(args.steps, 5))
+        extra_big = big_block(fa, torch, dev, min(args.steps, 3))
 
     cpu = None
     if not args.no_cpu:
-        import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        oracle.set_threads(threads)
-        o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, s.values)
-        o.factor()
-        times = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            xo, ro = o.solve(s.b, tol=tol, maxit=maxit)
-            times.append(time.perf_counter() - t0)
-        # the same oracle on one thread, over a bounded sample: 3 fixed iterations
-        oracle.set_threads(1)
-        t0 = time.perf_counter()
-        o.solve(s.b, tol=0.0, maxit=3)
-        per_it_1 = (time.perf_counter() - t0) / 3
-        oracle.set_threads(threads)
-        cpu = {"value": min(times), "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"full FP64 solve of {args.workload} to rel 1e-6 ({ro['iterations']} iterations), "
-                         f"best of 2, OpenMP over rows",
-               "s_per_iteration": min(times) / ro["iterations"],
-               "nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
-               "one_thread": {"s_per_iteration": per_it_1,
-                              "solve_s_estimate": per_it_1 * ro["iterations"],
-                              "sample": "3 fixed iterations (tol 0) on 1 thread, setup included"}}
-        log(f"cpu oracle: {times} s, iterations {ro['iterations']}; 1 thread {per_it_1:.3f} s per iteration")
-
+        cpu = cpu_baseline(fa, s, args, tol, maxit)
     out = {
         "metric": "time-to-1e-6 rel-res (s), mixed vs FP64 on GPU; achieved HBM GB/s",
         "value": t_mixed,
@@ -237,7 +217,7 @@ def main():
         "dtype": "f32 values/vectors, f64 sums+refinement",
         "data": "synthetic",
         "config": {"workload": args.workload, "grid": [spec.nx, spec.ny, spec.nz], "block": s.bs,
-                   "block_rows": s.n, "blocks": s.nnzb, "unknowns": s.N, "colours": g.num_colours,
+                   "block_rows": s.n, "blocks": s.nnzb, "unknowns": s.N, "colours": ncolours,
                    "tol": tol, "inner_tol": args.inner_tol},
         "fp64": {"time_s": t_fp64, "iterations": iters64, "rel_residual": r64.rel_residual,
                  "x_rel_err_vs_1e-12": err64, "iters_per_s": iters64 / t_fp64,
@@ -256,13 +236,194 @@ def main():
                      "frac": spmv32_gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"k_stream<{s.bs}, SM_SPMV, SR_W0, float, float> (the precision form's SpMV)",
                      "avg_launch_ms": spmv32_ms, "algorithmic_bytes_per_launch": spmv32_bytes},
-        "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac},
-        "adjoint": adj,
+        "trisolve": trisolve,
+        "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac, "fp32_copies": t_dem},
+        "adjoint": extra_adj,
+        "grid256x256x64": extra_big,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    g.close()
     return 0
+
+
+def iteration_bytes(g, fa, s, fp32):
+    """Algorithmic bytes of one BiCGSTAB iteration: 2 SpMV + 2 (L + U) + 22 N
+    vector element passes (the same for the transposed operators)."""
+    sx = 4 if fp32 else 8
+    return (2 * g.algorithmic_bytes(fa.KIND_SPMV, fp32)
+            + 2 * (g.algorithmic_bytes(fa.KIND_LOWER, fp32) + g.algorithmic_bytes(fa.KIND_UPPER, fp32))
+            + 22 * s.N * sx)
+
+
+def grad_bytes(s):
+    """The gradient pass: one FP64 value written per matrix entry, the block's
+    row and column indices and source slot read, lambda and x read."""
+    return s.nnzb * s.bs * s.bs * 8 + s.nnzb * 12 + 2 * s.N * 8
+
+
+def setup(fa, spec, torch):
+    t0 = time.perf_counter()
+    s = fa.generate(spec)
+    t_gen = time.perf_counter() - t0
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_system(s)
+    t0 = time.perf_counter()
+    g.analyse()
+    t_an = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.factor()
+    t_fac = time.perf_counter() - t0
+    return s, g, {"generate": t_gen, "analyse": t_an, "factor": t_fac}
+
+
+def adjoint_block(fa, torch, dev, steps):
+    """The 64x64x32 grid's backward pass: A^T lambda = w (w ~ N(0,1)) to
+    1e-10 with the transposed colour-ordered ILU0, then dL/dA = -lambda x^T."""
+    from fpga_amd.configs import GRID_64x64x32
+    s, g, st = setup(fa, GRID_64x64x32, torch)
+    b = torch.from_numpy(s.b).to(dev)
+    x = torch.zeros(s.N, dtype=torch.float64, device=dev)
+    g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-10, 1000)
+    w = torch.from_numpy(np.random.default_rng(0).standard_normal(s.N)).to(dev)
+    lam = torch.zeros_like(w)
+    grad = torch.empty((s.nnzb, s.bs, s.bs), dtype=torch.float64, device=dev)
+    g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, 1000)   # builds the transposed operators
+    times, ra = [], None
+    for _ in range(steps):
+        lam.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ra = g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, 1000)
+        g.grad_values_dev(lam.data_ptr(), x.data_ptr(), grad.data_ptr())
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    by = iteration_bytes(g, fa, s, False) * ra.iterations + grad_bytes(s)
+    out = {"workload": "grid64x64x32", "block_rows": s.n, "blocks": s.nnzb, "time_s": t, "tol": 1e-10,
+           "iterations": ra.iterations, "converged": ra.converged, "rel_residual": ra.rel_residual,
+           "GBs_algorithmic": by / t / 1e9, "steps": steps,
+           "note": "175 MB resident set: cache-resident, GB/s above HBM peak is possible"}
+    g.close()
+    log(f"adjoint grid64x64x32: {t * 1e3:.2f} ms, {ra.iterations} iterations")
+    return out
+
+
+def big_block(fa, torch, dev, steps):
+    """The 256x256x64 grid with 4x4 blocks: FP64 forward and precision-form
+    forward to 1e-6, adjoint backward to 1e-10 with the gradient; x errors
+    against a 1e-10 FP64 solve."""
+    from fpga_amd.configs import GRID_256x256x64
+    s, g, st = setup(fa, GRID_256x256x64, torch)
+    t0 = time.perf_counter()
+    g.prepare_mixed()
+    st["fp32_copies"] = time.perf_counter() - t0
+    b = torch.from_numpy(s.b).to(dev)
+    x = torch.zeros(s.N, dtype=torch.float64, device=dev)
+    g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-10, 2000)
+    x_ref = x.clone()
+    nref = torch.linalg.vector_norm(x_ref).item()
+
+    def timed(fn):
+        fn()
+        times, r = [], None
+        for _ in range(steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), r
+
+    t64, r64 = timed(lambda: g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-6, 1000))
+    e64 = torch.linalg.vector_norm(x - x_ref).item() / nref
+    tmx, rmx = timed(lambda: g.solve_mixed_dev(b.data_ptr(), x.data_ptr(), 1e-6, 20, 1e-4, 1000, True))
+    emx = torch.linalg.vector_norm(x - x_ref).item() / nref
+    w = torch.from_numpy(np.random.default_rng(0).standard_normal(s.N)).to(dev)
+    lam = torch.zeros_like(w)
+    grad = torch.empty((s.nnzb, s.bs, s.bs), dtype=torch.float64, device=dev)
+
+    def adj():
+        lam.zero_()
+        r = g.adjoint_solve_dev(w.data_ptr(), lam.data_ptr(), 1e-10, 1000)
+        g.grad_values_dev(lam.data_ptr(), x_ref.data_ptr(), grad.data_ptr())
+        return r
+
+    tad, rad = timed(adj)
+    it64, it32 = iteration_bytes(g, fa, s, False), iteration_bytes(g, fa, s, True)
+    out = {"workload": "grid256x256x64", "block": s.bs, "block_rows": s.n, "blocks": s.nnzb, "unknowns": s.N,
+           "colours": g.num_colours, "setup_s": st, "steps": steps,
+           "fp64": {"time_s": t64, "iterations": r64.iterations, "x_rel_err_vs_1e-10": e64,
+                    "GBs_algorithmic": it64 * r64.iterations / t64 / 1e9},
+           "mixed": {"time_s": tmx, "outer": rmx.outer_iterations, "inner_iterations": rmx.iterations,
+                     "x_rel_err_vs_1e-10": emx, "speedup_vs_fp64": t64 / tmx,
+                     "GBs_algorithmic_approx": it32 * rmx.iterations / tmx / 1e9},
+           "adjoint": {"time_s": tad, "tol": 1e-10, "iterations": rad.iterations, "converged": rad.converged,
+                       "GBs_algorithmic": (it64 * rad.iterations + grad_bytes(s)) / tad / 1e9}}
+    g.close()
+    log(f"grid256x256x64: fp64 {t64 * 1e3:.1f} ms, mixed {tmx * 1e3:.1f} ms, adjoint {tad * 1e3:.1f} ms")
+    return out
+
+
+def cpu_baseline(fa, s, args, tol, maxit):
+    """The CPU oracle (the same algorithm, OpenMP over the rows of a colour)
+    on this machine's host cores."""
+    import hashlib
+
+    import oracle
+    from fpga_amd.configs import GRID_32x32x16
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    o = oracle.OracleSystem(s.bs, s.rowptr, s.colidx, s.values)
+    o.factor()
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        xo, ro = o.solve(s.b, tol=tol, maxit=maxit)
+        times.append(time.perf_counter() - t0)
+    oracle.set_threads(1)
+    t0 = time.perf_counter()
+    _, r1 = o.solve(s.b, tol=tol, maxit=maxit)
+    t1 = time.perf_counter() - t0
+    del o
+    # the CPU-runnable case: 32x32x16, rel tol 1e-2, maxit 200
+    c1 = fa.generate(GRID_32x32x16)
+    res1 = {}
+    for nth in (threads, 1):
+        oracle.set_threads(nth)
+        t0 = time.perf_counter()
+        o1 = oracle.OracleSystem(c1.bs, c1.rowptr, c1.colidx, c1.values)
+        t_col = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o1.factor()
+        t_f = time.perf_counter() - t0
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            _, rc1 = o1.solve(c1.b, tol=1e-2, maxit=200)
+            ts.append(time.perf_counter() - t0)
+        colour, perm, _ = o1.colouring()
+        res1[f"threads_{nth}"] = {"solve_s_median": float(np.median(ts)), "solve_s": ts,
+                                  "colour_and_permute_s": t_col, "factor_s": t_f}
+        res1["iterations"] = rc1["iterations"]
+        res1["half_iterations"] = rc1["half_iterations"]
+        res1["rel_residual"] = rc1["rel_residual"]
+        res1["colour_sha256"] = hashlib.sha256(colour.tobytes()).hexdigest()
+        res1["perm_sha256"] = hashlib.sha256(perm.tobytes()).hexdigest()
+        del o1
+    oracle.set_threads(threads)
+    med = float(np.median(times))
+    log(f"cpu oracle: {times} s ({ro['iterations']} iterations), 1 thread {t1:.2f} s; "
+        f"grid32x32x16 {res1[f'threads_{threads}']['solve_s_median'] * 1e3:.2f} ms")
+    return {"value": med, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"full FP64 solve of {args.workload} to rel 1e-6 ({ro['iterations']} iterations), "
+                      f"median of 3, OpenMP over the rows of a colour",
+            "s_per_iteration": med / ro["iterations"],
+            "nproc": os.cpu_count(), "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "one_thread": {"solve_s": t1, "iterations": r1["iterations"],
+                           "sample": "one full FP64 solve to rel 1e-6 on 1 thread"},
+            "grid32x32x16": dict(res1, sample="FP64 solve to rel 1e-2, maxit 200, median of 5, "
+                                             "all threads and 1 thread; colouring SHA-256 of colour[] and perm[]")}
 
 
 def in_solve_avg(times_ms):

@@ -90,6 +90,7 @@ SIGNATURES = {
     "bda_num_colours": (i32, [vp, P_i32]),
     "bda_get_colouring": (i32, [vp, vp, vp, vp]),
     "bda_factor": (i32, [vp]),
+    "bda_prepare_mixed": (i32, [vp]),
     "bda_set_options": (i32, [vp, f64, i32, i32]),
     "bda_solve": (i32, [vp, vp, vp, f64, i32, C.POINTER(BdaResult)]),
     "bda_solve_dev": (i32, [vp, vp, vp, f64, i32, C.POINTER(BdaResult)]),
@@ -113,6 +114,7 @@ SIGNATURES = {
     "bda_get_kernel_stats": (i32, [vp, i32, C.POINTER(i64), P_f64]),
     "bda_get_kernel_times": (i32, [vp, i32, vp, i32, P_i32]),
     "bda_time_spmv": (i32, [vp, i32, i32, P_f64]),
+    "bda_time_precond": (i32, [vp, i32, i32, P_f64]),
     "bda_algorithmic_bytes": (i32, [vp, i32, i32, P_f64]),
     "bda_set_history": (i32, [vp, i32]),
     "bda_get_history": (i32, [vp, vp, i32, P_i32]),

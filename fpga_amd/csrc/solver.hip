@@ -1356,6 +1356,33 @@ double time_spmv(H &h, int reps, int fp32) {
     return reps > 0 ? ms / reps : 0.0;
 }
 
+// Average duration of one preconditioner application z = M^-1 p (every
+// launch of both sweeps) in the form the solves run, from reps back-to-back
+// applications bracketed by two HIP events on the handle's stream.
+template <int BS>
+double time_precond(H &h, int reps, int fp32) {
+    require_factored(h);
+    if (fp32) {
+        ensure_mixed<BS>(h);
+        h.k32.ensure(h.Npad());
+    }
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    for (int rep = -1; rep < reps; ++rep) {
+        if (rep == 0) HIP_CHECK(hipEventRecord(e0, h.stream));
+        if (fp32) precond<BS, float, float>(h, op_fwd32(h), h.k32.p.p, h.k32.y.p, h.k32.phat.p, false);
+        else precond<BS, double, double>(h, op_fwd64(h), h.k64.p.p, h.k64.y.p, h.k64.phat.p, false);
+    }
+    HIP_CHECK(hipEventRecord(e1, h.stream));
+    HIP_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return reps > 0 ? ms / reps : 0.0;
+}
+
 template <typename F>
 int guarded(bda_handle h, F &&f) {
     if (!h) return 2;
@@ -1548,6 +1575,10 @@ int bda_get_colouring(bda_handle h, int32_t *colour, int32_t *perm, int32_t *col
 
 int bda_factor(bda_handle h) {
     return guarded(h, [&] { BS_CALL(h, do_factor, *h); });
+}
+
+int bda_prepare_mixed(bda_handle h) {
+    return guarded(h, [&] { BS_CALL(h, ensure_mixed, *h); });
 }
 
 int bda_set_options(bda_handle h, double timeout_s, int32_t abs_tol, int32_t poll_iters) {
@@ -1835,6 +1866,15 @@ int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms) {
         double v = 0.0;
         if (h->bs == 3) v = time_spmv<3>(*h, reps, fp32);
         else v = time_spmv<4>(*h, reps, fp32);
+        if (avg_ms) *avg_ms = v;
+    });
+}
+
+int bda_time_precond(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms) {
+    return guarded(h, [&] {
+        double v = 0.0;
+        if (h->bs == 3) v = time_precond<3>(*h, reps, fp32);
+        else v = time_precond<4>(*h, reps, fp32);
         if (avg_ms) *avg_ms = v;
     });
 }

@@ -238,6 +238,10 @@ class BicgstabSolver:
     def factor(self):
         self._check(lib.bda_factor(self._h), "bda_factor")
 
+    def prepare_mixed(self):
+        """Make the precision form's FP32 copies of the factored system now."""
+        self._check(lib.bda_prepare_mixed(self._h), "bda_prepare_mixed")
+
     def set_options(self, timeout_s: float = 0.0, abs_tol: bool = False, poll_iters: int = 1):
         self._check(lib.bda_set_options(self._h, timeout_s, int(abs_tol), poll_iters), "bda_set_options")
 
@@ -391,6 +395,11 @@ class BicgstabSolver:
     def time_spmv(self, reps: int = 20, fp32: bool = False) -> float:
         ms = C.c_double()
         self._check(lib.bda_time_spmv(self._h, reps, int(fp32), C.byref(ms)), "bda_time_spmv")
+        return ms.value
+
+    def time_precond(self, reps: int = 20, fp32: bool = False) -> float:
+        ms = C.c_double()
+        self._check(lib.bda_time_precond(self._h, reps, int(fp32), C.byref(ms)), "bda_time_precond")
         return ms.value
 
     def algorithmic_bytes(self, kind: int, fp32: bool = False) -> float:

@@ -160,6 +160,11 @@ int bda_get_colouring(bda_handle h, int32_t *colour, int32_t *perm, int32_t *col
  * the reference receives L, U and the inverted diagonal blocks ready-made
  * (fpga_functions_bicgstab.cpp:615-617,629). */
 int bda_factor(bda_handle h);
+/* Makes the FP32 copies of A, L and U that the precision form runs on, from
+ * the current factorisation.  They are otherwise made by the first
+ * precision-form call after each bda_factor; this call lets a caller do
+ * (and time) that step with the factorisation.  No reference counterpart. */
+int bda_prepare_mixed(bda_handle h);
 
 /* Host watchdog in seconds (0 = off), counterpart of param0's abort cycles
  * (bicgstab_solver_wrapper_2r_3r3w_simple-ddr_v1.vhd:605-643); abs_tol != 0
@@ -369,6 +374,11 @@ int bda_get_kernel_times(bda_handle h, int32_t kind, float *ms, int32_t capacity
  * fp32 != 0) on device-resident vectors; average launch time in ms from two
  * HIP events on the handle's stream. */
 int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
+/* reps back-to-back preconditioner applications z = M^-1 p (all launches of
+ * the L and U sweeps; FP64, or the precision form's FP32 values and vectors
+ * with fp32 != 0) on device-resident vectors; average time of one
+ * application in ms from two HIP events on the handle's stream. */
+int bda_time_precond(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
 /* Algorithmic bytes of one launch of the kernel classes above (FP64 form). */
 int bda_algorithmic_bytes(bda_handle h, int32_t kind, int32_t fp32, double *bytes);
 

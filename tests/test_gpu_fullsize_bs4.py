@@ -56,3 +56,31 @@ def test_full_bs4_solve_parity(big4):
     import scipy.sparse as sp
     A = sp.bsr_matrix((s.values, s.colidx, s.rowptr), shape=(s.N, s.N))
     assert np.linalg.norm(s.b - A @ xg) / np.linalg.norm(s.b) < 1.1e-6
+
+
+def test_full_bs4_precision_form_and_adjoint(big4):
+    """The 256x256x64 4x4 workload end to end: the precision form to 1e-6
+    (FP32 values and inner vectors, FP64 refinement) with an x error no worse
+    than the FP64 solve's at the same tolerance, both measured against a
+    1e-10 FP64 solve; the adjoint A^T lambda = w to 1e-10 within 1e-8 of the
+    oracle's and within one iteration of its count; the gradient -lambda x^T
+    on the pattern bit for bit."""
+    s, g, o = big4
+    x_ref, r_ref = g.solve(s.b, tol=1e-10, maxit=2000)
+    assert r_ref.converged
+    xm, rm = g.solve_mixed(s.b, tol=1e-6, inner_tol=1e-4, vec32=True)
+    xf, rf = g.solve(s.b, tol=1e-6, maxit=1000)
+    nref = np.linalg.norm(x_ref)
+    err_m, err_f = np.linalg.norm(xm - x_ref) / nref, np.linalg.norm(xf - x_ref) / nref
+    print(f"grid256x256x64 bs4: mixed {rm.outer_iterations} outer / {rm.iterations} inner it, x err {err_m:.3e}; "
+          f"fp64 {rf.iterations} it, x err {err_f:.3e}", flush=True)
+    assert rm.converged and rf.converged
+    assert err_m <= err_f
+    w = np.random.default_rng(12).standard_normal(s.N)
+    lg, rg = g.adjoint_solve(w, tol=1e-10, maxit=1000)
+    lo, ro = o.adjoint_solve(w, tol=1e-10, maxit=1000)
+    print(f"adjoint half iterations gpu {rg.half_iterations} oracle {ro['half_iterations']}", flush=True)
+    assert rg.converged and ro["converged"]
+    assert abs(rg.half_iterations - ro["half_iterations"]) <= 2
+    assert np.linalg.norm(lg - lo) / np.linalg.norm(lo) < 1e-8
+    assert h(g.grad_values(lo, x_ref)) == h(o.grad(lo, x_ref))
