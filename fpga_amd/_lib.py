@@ -86,6 +86,7 @@ SIGNATURES = {
     "bda_update_values": (i32, [vp, vp]),
     "bda_update_values_dev": (i32, [vp, vp]),
     "bda_values_generation": (i32, [vp, C.POINTER(C.c_uint64)]),
+    "bda_upload_factored": (i32, [vp, i32, vp, vp, vp, vp, vp, vp]),
     "bda_analyse": (i32, [vp, i32]),
     "bda_num_colours": (i32, [vp, P_i32]),
     "bda_get_colouring": (i32, [vp, vp, vp, vp]),
@@ -130,7 +131,20 @@ SIGNATURES = {
 }
 
 
+def _bind_torch_runtime():
+    """torch and this library both run on the HIP runtime (soname
+    libamdhip64.so.7).  torch loads its own bundled copy under another name,
+    so if the library were loaded first the process would hold two runtimes
+    and torch would see no GPU.  When torch is installed it is imported
+    first; the library's dependency then resolves to torch's copy."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path: Path | str | None = None) -> C.CDLL:
+    _bind_torch_runtime()
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise RuntimeError(

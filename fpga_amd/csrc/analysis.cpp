@@ -134,6 +134,39 @@ void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {
     an.bs = m.bs;
     an.level_scheduling = level_scheduling;
     an.ncolours = colour_rows(m, level_scheduling, an.colour);
+    analyse_coloured(m, an);
+}
+
+void analyse_given_order(const Bcsr &m, const std::vector<int32_t> &colour_ptr, Analysis &an) {
+    const int n = m.n;
+    if (n <= 0) throw std::runtime_error("empty matrix");
+    if ((int)m.rowptr.size() != n + 1 || m.rowptr[0] != 0) throw std::runtime_error("bad rowptr");
+    const int nc = (int)colour_ptr.size() - 1;
+    if (nc < 1 || colour_ptr[0] != 0 || colour_ptr[nc] != n)
+        throw std::runtime_error("colour_ptr must run from 0 to n");
+    for (int c = 0; c < nc; ++c)
+        if (colour_ptr[c + 1] <= colour_ptr[c]) throw std::runtime_error("every colour must hold at least one row");
+    for (int i = 0; i < n; ++i) {
+        if (m.rowptr[i + 1] < m.rowptr[i]) throw std::runtime_error("rowptr not monotone");
+        for (int32_t q = m.rowptr[i]; q < m.rowptr[i + 1]; ++q) {
+            const int32_t j = m.colidx[q];
+            if (j < 0 || j >= n) throw std::runtime_error("column index out of range in row " + std::to_string(i));
+            if (q > m.rowptr[i] && m.colidx[q - 1] >= j)
+                throw std::runtime_error("column indices not strictly increasing in row " + std::to_string(i));
+        }
+    }
+    an.n = n;
+    an.bs = m.bs;
+    an.level_scheduling = false;
+    an.ncolours = nc;
+    an.colour.assign(n, 0);
+    for (int c = 0; c < nc; ++c)
+        for (int i = colour_ptr[c]; i < colour_ptr[c + 1]; ++i) an.colour[i] = c;
+    analyse_coloured(m, an);   // the colours are non-decreasing in row order: the permutation is the identity
+}
+
+void analyse_coloured(const Bcsr &m, Analysis &an) {
+    const int n = m.n;
     const int ncol = an.ncolours;
 
     // stable counting sort by colour

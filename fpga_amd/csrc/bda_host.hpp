@@ -74,6 +74,12 @@ int colour_rows(const Bcsr &m, bool level_scheduling, std::vector<int32_t> &colo
 
 // Full analysis; throws std::runtime_error on a bad pattern.
 void analyse(const Bcsr &m, bool level_scheduling, Analysis &an);
+// The same for a matrix whose rows are already in colour order, colour c
+// holding rows [colour_ptr[c], colour_ptr[c+1]): no colouring, identity
+// permutation; throws when two rows of one colour are coupled.
+void analyse_given_order(const Bcsr &m, const std::vector<int32_t> &colour_ptr, Analysis &an);
+// The part of the analysis after the colouring (an.colour, an.ncolours set).
+void analyse_coloured(const Bcsr &m, Analysis &an);
 
 // Build tiles for pattern p whose rows are grouped by colour_ptr.
 void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs);

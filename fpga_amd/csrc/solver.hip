@@ -852,9 +852,9 @@ void permute_values(H &h) {
     HIP_CHECK(hipStreamSynchronize(h.stream));
 }
 
-void do_analyse(H &h, bool level) {
-    if (!h.have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
-    analyse(h.pattern, level, h.an);
+// Device side of an analysis held in h.an: patterns, tilings, permutation
+// and the buffers sized by them.
+void finish_analyse(H &h) {
     const Analysis &an = h.an;
     const int64_t span = DevMat::coupling_span(an.A, an.perm);
     h.A.upload(an.A, h.stream, &an.perm, span);
@@ -904,6 +904,12 @@ void do_analyse(H &h, bool level) {
     }
 }
 
+void do_analyse(H &h, bool level) {
+    if (!h.have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
+    analyse(h.pattern, level, h.an);
+    finish_analyse(h);
+}
+
 template <int BS>
 void do_factor(H &h) {
     require_analysed(h);
@@ -932,6 +938,31 @@ void do_factor(H &h) {
     if (bad != INT_MAX)
         throw std::runtime_error("singular diagonal block in ILU0 at colour-ordered row " + std::to_string(bad) +
                                  " (original row " + std::to_string(an.perm[bad]) + ")");
+    gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
+    gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
+    gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+    h.factored = true;
+    h.factor_gen++;
+    h.values_gen++;
+}
+
+// The caller's own factors of a colour-ordered system (identity
+// permutation): L and U from lu_values on A's pattern, D^-1 as given.  The
+// finiteness flag of the step-0 short path is set from A as bda_factor
+// sets it.
+template <int BS>
+void load_factors(H &h, const double *lu_values, const double *dinv) {
+    constexpr int BB = BS * BS;
+    const int32_t zero = 0;
+    HIP_CHECK(hipMemcpyAsync(h.d_nonfinite.p, &zero, 4, hipMemcpyHostToDevice, h.stream));
+    const int64_t nval = (int64_t)h.A.nnzb * BB;
+    const int grid = (int)std::min<int64_t>(std::max<int64_t>(blocks_for(nval, 256), 1), 8192);
+    hipLaunchKernelGGL(k_copy_check_finite, dim3(grid), dim3(256), 0, h.stream, nval, (const double *)h.A64.p, h.LU.p,
+                       h.d_nonfinite.p);
+    check_launch();
+    HIP_CHECK(hipMemcpyAsync(h.LU.p, lu_values, (size_t)nval * 8, hipMemcpyHostToDevice, h.stream));
+    HIP_CHECK(hipMemcpyAsync(h.Dinv.p, dinv, (size_t)h.n * BB * 8, hipMemcpyHostToDevice, h.stream));
     gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
     gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
     gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
@@ -1543,6 +1574,36 @@ int bda_update_values_dev(bda_handle h, const double *d_values) {
         h->factored = false;
         h->values_gen++;
         if (h->analysed) BS_CALL(h, permute_values, *h);
+    });
+}
+
+int bda_upload_factored(bda_handle h, int32_t ncolours, const int32_t *colour_ptr, const int32_t *rowptr,
+                        const int32_t *colidx, const double *values, const double *lu_values, const double *dinv) {
+    return guarded(h, [&] {
+        if (!colour_ptr || !rowptr || !colidx || !values || !lu_values || !dinv)
+            throw std::runtime_error("null array");
+        if (ncolours < 1 || ncolours > h->n) throw std::runtime_error("ncolours must be in [1, n]");
+        if (rowptr[0] != 0) throw std::runtime_error("rowptr[0] must be 0");
+        for (int i = 0; i < h->n; ++i)
+            if (rowptr[i + 1] < rowptr[i] || rowptr[i + 1] > h->nnzb_max)
+                throw std::runtime_error("rowptr[" + std::to_string(i + 1) + "] is decreasing or exceeds nnzb_max");
+        const int64_t nnzb = rowptr[h->n];
+        Bcsr m;
+        m.bs = h->bs;
+        m.n = h->n;
+        m.rowptr.assign(rowptr, rowptr + h->n + 1);
+        m.colidx.assign(colidx, colidx + nnzb);
+        const std::vector<int32_t> cptr(colour_ptr, colour_ptr + ncolours + 1);
+        Analysis an;
+        analyse_given_order(m, cptr, an);   // validates the pattern and the colouring before any copy
+        h->pattern = m;
+        h->have_pattern = true;
+        h->an = std::move(an);
+        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        h->have_values = true;
+        h->values_gen++;
+        finish_analyse(*h);   // identity permutation: A's values are copied as given
+        BS_CALL(h, load_factors, *h, lu_values, dinv);
     });
 }
 

@@ -205,6 +205,15 @@ class BicgstabSolver:
     def upload_system(self, sysm: BlockSystem):
         self.upload_bcsr(sysm.rowptr, sysm.colidx, sysm.values)
 
+    def upload_factored(self, colour_ptr, rowptr, colidx, values, lu_values, dinv):
+        """A colour-ordered system with the caller's own L, U (on A's pattern)
+        and D^-1; the handle then solves with exactly these factors."""
+        cp = _i32(colour_ptr)
+        self._rowptr, self._colidx = _i32(rowptr), _i32(colidx)
+        v, lu, d = _f64(values), _f64(lu_values), _f64(dinv)
+        self._check(lib.bda_upload_factored(self._h, cp.size - 1, _p(cp), _p(self._rowptr), _p(self._colidx),
+                                            _p(v), _p(lu), _p(d)), "bda_upload_factored")
+
     def update_values(self, values):
         v = _f64(values)
         self._check(lib.bda_update_values(self._h, _p(v)), "bda_update_values")

@@ -144,6 +144,22 @@ int bda_update_values_dev(bda_handle h, const double *d_values);
  * (the reference's buffers are rewritten by its caller only). */
 int bda_values_generation(bda_handle h, uint64_t *gen);
 
+/* The caller's own colour-ordered system with its own factors, as the
+ * reference kernel receives them from its caller (fpga_copy_host_datamem,
+ * fpga_functions_bicgstab.cpp:606-629; the colour boundaries of
+ * rtl/bicgstab_solver_2r-3r3w/sparstition.vhd:673-717): rows already in
+ * colour order, colour c holding rows [colour_ptr[c], colour_ptr[c+1])
+ * (rows of one colour must not be coupled), A in block CSR (sorted columns,
+ * diagonal present), lu_values[nnzb*bs*bs] on A's pattern holding L on the
+ * strictly lower blocks (unit diagonal implicit) and U on the strictly upper
+ * ones (diagonal blocks ignored), dinv[n*bs*bs] the inverted diagonal blocks
+ * of U.  Nothing is coloured, reordered or factorised: the solves, the
+ * adjoint and the component access then run on exactly these factors, in the
+ * given numbering (the permutation is the identity).  bda_wire_import turns
+ * the reference's per-colour device arrays into these arrays. */
+int bda_upload_factored(bda_handle h, int32_t ncolours, const int32_t *colour_ptr, const int32_t *rowptr,
+                        const int32_t *colidx, const double *values, const double *lu_values, const double *dinv);
+
 /* Greedy colouring + colour-ordered permutation (host).  Counterpart of the
  * colour sizes / P-index arrays the reference consumes
  * (fpga_functions_bicgstab.cpp:606-613).  level_scheduling != 0 selects

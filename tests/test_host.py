@@ -102,3 +102,15 @@ def test_config2_colour_count_is_stable():
     _, _, cp = fa.colour_pattern(s.rowptr, s.colidx)
     assert (s.n, s.nnzb) == (943550, 6003904)
     assert list(np.diff(cp)) == [449499, 448480, 36644, 8905, 22]
+
+
+def test_one_hip_runtime_per_process():
+    """Importing the package before torch must not leave two HIP runtimes in
+    the process (torch would then see no GPU): the package binds torch's."""
+    import subprocess
+    import sys
+    code = ("import fpga_amd, torch\n"
+            "print(len(set(l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l)))")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=str(ROOT))
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "1", out.stdout
