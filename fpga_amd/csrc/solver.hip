@@ -671,17 +671,25 @@ bool run_stopped(const H &h, int idx, int32_t run_id) {
 
 // Waits until run run_id on state slot idx has reported progress >= count,
 // or has stopped; false if the handle's timeout (measured from t0) expired.
+// The host polls the mapped words; a wait that has lasted longer than
+// kSpinWait (the device is more than a short kernel sequence behind: large
+// systems) polls every kSleepStep instead, so that the waiting thread gives
+// its core back (a simulator runs one MPI rank per core).  Short waits, which
+// decide how soon the next launches are queued, keep spinning.
+constexpr std::chrono::microseconds kSpinWait{200}, kSleepStep{20};
+
 bool wait_progress(const H &h, int idx, int32_t run_id, int32_t count,
                    std::chrono::steady_clock::time_point t0) {
+    const auto w0 = std::chrono::steady_clock::now();
     for (uint32_t spin = 0;; ++spin) {
         const uint64_t v = __atomic_load_n(h.h_prog + idx, __ATOMIC_ACQUIRE);
         if ((int32_t)(v >> 32) == run_id && (int32_t)(uint32_t)v >= count) return true;
         if (run_stopped(h, idx, run_id)) return true;
-        if ((spin & 1023) == 1023) {
-            if (h.timeout_s > 0.0 &&
-                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > h.timeout_s)
-                return false;
-            std::this_thread::yield();
+        if ((spin & 255) == 255) {
+            const auto now = std::chrono::steady_clock::now();
+            if (h.timeout_s > 0.0 && std::chrono::duration<double>(now - t0).count() > h.timeout_s) return false;
+            if (now - w0 > kSpinWait) std::this_thread::sleep_for(kSleepStep);
+            else std::this_thread::yield();
         }
     }
 }

@@ -530,12 +530,68 @@ int bda_wire_unpack(uint8_t *const buffers[5], int32_t vector_sizes[18], int32_t
 // Decoding into colour-ordered BCSR.  A's block pattern is the union of the
 // blocks its stream touches; L and U values are read on A's strictly lower
 // and upper blocks (entries outside them must be -0.0 or 0.0 fillers).
+}  // extern "C"
+
+namespace {
+// Checks every stream of a foreign buffer image before anything is decoded:
+// partition sizes non-negative, each entry's row inside its partition, its
+// slot inside the partition's gathered list, every gathered index inside the
+// vector, the running offsets inside the arrays vector_sizes declares, the
+// partitions' rows within N (covering it exactly for A), and D^-1's rows and
+// values inside their array.
+void validate_streams(int32_t bs, const int32_t vector_sizes[18], void *const vector_pointers[21]) {
+    if (bs != 3 && bs != 4) throw std::runtime_error("bad block size");
+    const int32_t N = vector_sizes[0];
+    if (N <= 0 || N % bs) throw std::runtime_error("row count is not a positive multiple of the block size");
+    for (int k = 0; k < 3; ++k) {
+        const int base = 6 * k;
+        const int32_t parts = vector_sizes[base + 2];
+        const int64_t n_slots = vector_sizes[base + 1], n_p = vector_sizes[base + 3], n_nr = vector_sizes[base + 4];
+        if (parts < 0 || n_slots < 0 || n_p < 0 || n_nr < 0) throw std::runtime_error("negative array size");
+        if (parts > 0 && (!vector_pointers[base] || !vector_pointers[base + 1] || !vector_pointers[base + 3] ||
+                          !vector_pointers[base + 4]))
+            throw std::runtime_error("missing stream array");
+        const int32_t *cs = parts > 0 ? (const int32_t *)vector_pointers[base] + kHeaderWords : nullptr;
+        const uint32_t *P = (const uint32_t *)vector_pointers[base + 1];
+        const uint16_t *Ci = (const uint16_t *)vector_pointers[base + 3];
+        const uint8_t *NR = (const uint8_t *)vector_pointers[base + 4];
+        int64_t row0 = 0, op = 0, ov = 0, onr = 0;
+        for (int j = 0; j < parts; ++j) {
+            const int32_t arow = cs[4 * j + 1], col = cs[4 * j + 2], val = cs[4 * j + 3];
+            if (arow < 0 || col < 0 || val < 0) throw std::runtime_error("negative partition size");
+            if (op + col > n_p || ov + val > n_slots || onr + val > n_nr)
+                throw std::runtime_error("partition beyond its stream arrays");
+            if (row0 + arow > N) throw std::runtime_error("partition rows beyond the vector");
+            for (int32_t q = 0; q < col; ++q)
+                if (P[op + q] >= (uint32_t)N) throw std::runtime_error("P index outside the vector");
+            int64_t rr = -1;
+            for (int32_t q = 0; q < val; ++q) {
+                rr += NR[onr + q];
+                if (rr < 0 || rr >= arow) throw std::runtime_error("stream row outside the partition");
+                if (Ci[ov + q] >= col) throw std::runtime_error("column slot outside the gathered list");
+            }
+            row0 += arow;
+            op += round_up(col, kPAlign);
+            ov += val;
+            onr += round_up(val, kNRAlign);
+        }
+        if (k == KA && row0 != N) throw std::runtime_error("partition rows do not cover the vector");
+        if (k == KU) {
+            int64_t od = 0;
+            for (int j = 0; j < parts; ++j) od += round_up((int64_t)kDiagPerRow * cs[4 * j + 1], kDiagAlign);
+            if (od > vector_sizes[5]) throw std::runtime_error("D^-1 rows beyond their array");
+        }
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int bda_wire_import_sizes(int32_t bs, const int32_t vector_sizes[18], void *const vector_pointers[21], int32_t *n,
                           int64_t *nnzb, int32_t *num_parts) {
     return wire_guard([&] {
-        if (bs != 3 && bs != 4) throw std::runtime_error("bad block size");
+        validate_streams(bs, vector_sizes, vector_pointers);
         const int32_t N = vector_sizes[0];
-        if (N % bs) throw std::runtime_error("row count is not a multiple of the block size");
         const int nb = N / bs;
         const int32_t *cs = (const int32_t *)vector_pointers[0] + kHeaderWords;
         const uint32_t *P = (const uint32_t *)vector_pointers[1];
@@ -575,6 +631,7 @@ int bda_wire_import(int32_t bs, const int32_t vector_sizes[18], void *const vect
                     int32_t *colidx, double *values, double *lu_values, double *dinv, int32_t *part_rows,
                     double *b, double *x0) {
     return wire_guard([&] {
+        validate_streams(bs, vector_sizes, vector_pointers);
         const int32_t N = vector_sizes[0];
         const int nb = N / bs;
         const int bb = bs * bs;

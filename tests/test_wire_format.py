@@ -254,3 +254,36 @@ def test_device_fit_flag():
     assert small.device_fits
     big = setup_case(fa.GridSpec(40, 40, 40, bs=3, seed=0))[-1]   # N = 192,000 > 163,840 vector entries
     assert not big.device_fits
+
+
+def _copy(w):
+    c = wire.WireArrays(list(w.sizes), list(w.nnz_val_sizes), w.device_fits)
+    c.arrays = {k: v.copy() for k, v in w.arrays.items()}
+    return c
+
+
+@pytest.mark.parametrize("damage", ["L_slot", "U_slot", "L_p", "U_row", "U_parts", "dinv", "bs"])
+def test_import_rejects_corrupted_streams(case, damage):
+    """A foreign or damaged buffer image is refused before anything is
+    decoded: every stream's rows, slots and gathered indices, the partition
+    counts against the arrays and D^-1's extent are checked, not only A's."""
+    s, o, perm, cptr, A, (L, U, dinv), bp, xp, w = case
+    c = _copy(w)
+    bs = s.bs
+    if damage in ("L_slot", "U_slot"):
+        k = 1 if damage == "L_slot" else 2
+        col = c.colour_sizes(k)[0][2]
+        c.arrays[6 * k + 3][0] = col           # slot one past the partition's gathered list
+    elif damage == "L_p":
+        c.arrays[6 * 1 + 1][0] = s.N           # gathered index outside the vector
+    elif damage == "U_row":
+        c.arrays[6 * 2 + 4][0] = 250           # row advance beyond the partition
+    elif damage == "U_parts":
+        c.sizes[6 * 2 + 2] += 1                # one partition more than the header holds
+        c.arrays[6 * 2 + 0] = np.concatenate([c.arrays[12], np.array([0, 1 << 20, 0, 0], np.int32)])
+    elif damage == "dinv":
+        c.sizes[5] = 8                          # D^-1 array shorter than its rows
+    elif damage == "bs":
+        bs = 5
+    with pytest.raises(RuntimeError):
+        wire.import_arrays(bs, c)
