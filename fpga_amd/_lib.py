@@ -116,6 +116,7 @@ SIGNATURES = {
     "bda_get_kernel_times": (i32, [vp, i32, vp, i32, P_i32]),
     "bda_time_spmv": (i32, [vp, i32, i32, P_f64]),
     "bda_time_precond": (i32, [vp, i32, i32, P_f64]),
+    "bda_measure_bandwidth": (i32, [vp, i64, i32, P_f64, P_f64]),
     "bda_algorithmic_bytes": (i32, [vp, i32, i32, P_f64]),
     "bda_set_history": (i32, [vp, i32]),
     "bda_get_history": (i32, [vp, vp, i32, P_i32]),

@@ -405,11 +405,6 @@ struct StreamArgs {
     const double *dinv; // SUB_DINV: out = dinv * (rhs - A x)
     const TX *w0;       // SR_W0 reductions: sum out * w0
     RedArgs ra;
-    // SUB_DINV launch that runs both sweeps of a colour at once (the first
-    // sweep's last colour, whose rows have no second-sweep blocks): the first
-    // sweep's result rhs - A x is rounded to TX, as the vector the second
-    // sweep would read, before D^-1 is applied
-    int32_t round_mid;
 };
 
 template <typename TV, typename TX>
@@ -645,12 +640,6 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
         }
     }
     if constexpr (MODE == SM_SUB_DINV) {
-        if constexpr (!std::is_same<TX, double>::value) {
-            if (a.round_mid) {
-#pragma unroll
-                for (int s = 0; s < RS; ++s) res[s] = (double)from_double<TX>(res[s]);
-            }
-        }
 #pragma unroll
         for (int s = 0; s < RS; ++s) tv[s * NT + t] = res[s];
         __syncthreads();
@@ -1035,6 +1024,37 @@ template <typename TXX>
 __global__ __launch_bounds__(kVecThreads) void k_add_correction(int64_t N, double *x, const TXX *d) {
     const int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x;
     if (i < N) x[i] = x[i] + (double)d[i];
+}
+
+// ---------------------------------------------------------------------------
+// Bandwidth references (measurement only): a streaming read and a streaming
+// copy of 16-byte words, one pass, each thread kBwUnroll words kBwThreads
+// apart with all its loads in flight together (nontemporal, as the matrix
+// streams); the read writes one sum per wave so that its loads are kept.
+// n4 must be a multiple of kBwThreads * kBwUnroll.
+constexpr int kBwThreads = 256, kBwUnroll = 8;
+typedef double BwVec __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kBwThreads) void k_bw_read(const double2 *src2, double *out) {
+    const BwVec *src = reinterpret_cast<const BwVec *>(src2) + (size_t)blockIdx.x * kBwThreads * kBwUnroll +
+                       threadIdx.x;
+    BwVec v[kBwUnroll];
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) v[u] = __builtin_nontemporal_load(&src[u * kBwThreads]);
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) acc += v[u].x + v[u].y;
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) out[blockIdx.x * (kBwThreads / 64) + (threadIdx.x >> 6)] = acc;
+}
+__global__ __launch_bounds__(kBwThreads) void k_bw_copy(const double2 *src2, double2 *dst2) {
+    const size_t base = (size_t)blockIdx.x * kBwThreads * kBwUnroll + threadIdx.x;
+    const BwVec *src = reinterpret_cast<const BwVec *>(src2) + base;
+    BwVec *dst = reinterpret_cast<BwVec *>(dst2) + base;
+    BwVec v[kBwUnroll];
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) v[u] = __builtin_nontemporal_load(&src[u * kBwThreads]);
+#pragma unroll
+    for (int u = 0; u < kBwUnroll; ++u) __builtin_nontemporal_store(v[u], &dst[u * kBwThreads]);
 }
 
 // ---------------------------------------------------------------------------

@@ -449,8 +449,7 @@ int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
 template <int BS, int MODE, int RED, typename TV, typename TX, bool ZX = false>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
                    const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
-                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1,
-                   bool round_mid = false) {
+                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
     constexpr bool wide = wide_tiles<MODE>();
@@ -493,7 +492,6 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     a.dinv = dinv;
     a.w0 = w0;
     a.ra = ra;
-    a.round_mid = round_mid ? 1 : 0;
     if (!use_state) a.ra.st = h.st.p + kIdleState;
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
@@ -536,11 +534,11 @@ int vec_blocks(size_t N) { return (int)((N + kVecElemsPerBlock - 1) / kVecElemsP
 
 template <int BS, typename TV, typename TX>
 void sweep_launch(H &h, const DevMat &m, const TV *val, int c, bool dinv_mode, const TX *x, const TX *x_lo, int split,
-                  const TX *rhs, TX *out, const double *dinv, bool use_state, bool round_mid = false) {
+                  const TX *rhs, TX *out, const double *dinv, bool use_state) {
     RedArgs ra = red(h, 0, FIN_NONE);
     if (dinv_mode)
         launch_stream<BS, SM_SUB_DINV, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, dinv, nullptr, ra, use_state,
-                                                        nullptr, nullptr, x_lo, split, -1, round_mid);
+                                                        nullptr, nullptr, x_lo, split);
     else
         launch_stream<BS, SM_SUB, SR_NONE, TV, TX>(h, m, val, c, x, rhs, out, nullptr, nullptr, ra, use_state, nullptr,
                                                    nullptr, x_lo, split);
@@ -579,7 +577,6 @@ void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split
     a.out = z;
     a.dinv = op.dinv;
     a.ra = red(h, 0, FIN_NONE);
-    a.round_mid = op.s2_dinv ? 1 : 0;  // the first sweep's rows of colour C-1 rounded before the second's D^-1
     if (!use_state) a.ra.st = h.st.p + kIdleState;
     TailArgs<TV, TX> b;
     b.rowptr = m2.rowptr.p;
@@ -608,7 +605,8 @@ void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split
 //    skipped and later gathers read p for colour-0 columns;
 //  - the last colour has no blocks in the second sweep (nothing follows
 //    it), so its first-sweep launch writes z directly, applying the second
-//    sweep's diagonal inverse in the same epilogue when it has one.
+//    sweep's diagonal inverse in the same epilogue when it has one, to the
+//    first sweep's value before any rounding (that value is never stored).
 // When the last colour is a single tile and few rows of colour C-2 have
 // second-sweep blocks, that launch and the second sweep's colour C-2 launch
 // are one (tail_launch).
@@ -626,7 +624,7 @@ void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_stat
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
     if (!merge)
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
-                                 use_state, op.s2_dinv);
+                                 use_state);
     prof_end(h);
     prof_begin(h, PK_UPPER);
     if (merge) tail_launch<BS, TV, TX>(h, op, y, lo, split, p, z, use_state);
@@ -1945,6 +1943,48 @@ int bda_time_precond(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms) {
         if (h->bs == 3) v = time_precond<3>(*h, reps, fp32);
         else v = time_precond<4>(*h, reps, fp32);
         if (avg_ms) *avg_ms = v;
+    });
+}
+
+int bda_measure_bandwidth(bda_handle h, int64_t bytes, int32_t reps, double *read_gbs, double *copy_gbs) {
+    return guarded(h, [&] {
+        if (bytes < (1 << 20) || reps < 1) throw std::runtime_error("bytes >= 1 MiB and reps >= 1");
+        constexpr int64_t chunk = (int64_t)kBwThreads * kBwUnroll;   // 16-byte words per workgroup
+        const int64_t n4 = bytes / 16 / chunk * chunk;
+        DBuf<double2> a, b;
+        DBuf<double> part;
+        a.ensure((size_t)n4);
+        b.ensure((size_t)n4);
+        const int grid = (int)(n4 / chunk);
+        part.ensure((size_t)grid * (kBwThreads / 64));
+        hipEvent_t e0, e1;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        auto timed = [&](bool copy) {
+            for (int rep = -1; rep < reps; ++rep) {
+                if (rep == 0) HIP_CHECK(hipEventRecord(e0, h->stream));
+                if (copy)
+                    hipLaunchKernelGGL(k_bw_copy, dim3(grid), dim3(kBwThreads), 0, h->stream, (const double2 *)a.p,
+                                       b.p);
+                else
+                    hipLaunchKernelGGL(k_bw_read, dim3(grid), dim3(kBwThreads), 0, h->stream, (const double2 *)a.p,
+                                       part.p);
+                check_launch();
+            }
+            HIP_CHECK(hipEventRecord(e1, h->stream));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        const double ms_r = timed(false), ms_c = timed(true);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (read_gbs) *read_gbs = (double)n4 * 16 / (ms_r * 1e-3) / 1e9;
+        if (copy_gbs) *copy_gbs = 2.0 * (double)n4 * 16 / (ms_c * 1e-3) / 1e9;
+        a.release();
+        b.release();
+        part.release();
     });
 }
 

@@ -411,6 +411,12 @@ class BicgstabSolver:
         self._check(lib.bda_time_precond(self._h, reps, int(fp32), C.byref(ms)), "bda_time_precond")
         return ms.value
 
+    def measure_bandwidth(self, nbytes: int = 1 << 30, reps: int = 10):
+        """(read GB/s, copy GB/s) of streaming kernels over nbytes."""
+        r, c = C.c_double(), C.c_double()
+        self._check(lib.bda_measure_bandwidth(self._h, nbytes, reps, C.byref(r), C.byref(c)), "bda_measure_bandwidth")
+        return r.value, c.value
+
     def algorithmic_bytes(self, kind: int, fp32: bool = False) -> float:
         v = C.c_double()
         self._check(lib.bda_algorithmic_bytes(self._h, kind, int(fp32), C.byref(v)), "bda_algorithmic_bytes")

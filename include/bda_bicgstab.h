@@ -395,6 +395,14 @@ int bda_time_spmv(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
  * with fp32 != 0) on device-resident vectors; average time of one
  * application in ms from two HIP events on the handle's stream. */
 int bda_time_precond(bda_handle h, int32_t reps, int32_t fp32, double *avg_ms);
+/* The device's practical streaming bandwidth, for reading measured rates
+ * against: a one-pass read of `bytes` (rounded down to 32 KiB; 16-byte
+ * nontemporal loads, eight per thread in flight together, one sum per wave
+ * written) and a copy of the same size, reps launches each after one
+ * warm-up, timed with HIP events on the handle's stream; GB/s = bytes moved
+ * / average time.  Uses two scratch buffers of `bytes` for the call.  A
+ * measurement aid of this build, no reference counterpart. */
+int bda_measure_bandwidth(bda_handle h, int64_t bytes, int32_t reps, double *read_gbs, double *copy_gbs);
 /* Algorithmic bytes of one launch of the kernel classes above (FP64 form). */
 int bda_algorithmic_bytes(bda_handle h, int32_t kind, int32_t fp32, double *bytes);
 

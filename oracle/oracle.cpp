@@ -127,6 +127,39 @@ void sweep(const BMat<TV> &S, const std::vector<int32_t> &cptr, bool ascending, 
     }
 }
 
+// z = M^-1 p with M the two sweeps (S1 colours ascending into y, S2
+// descending into z; at most one of them carries D^-1).  A stored vector is
+// rounded to TX.  When only the second sweep carries D^-1, the last colour's
+// first-sweep value is not stored: that colour has no second-sweep blocks
+// (they point to higher colours), so its second-sweep step is the diagonal
+// solve alone and is applied to the unrounded value, z_i = D_i (p_i - S1 y)_i.
+template <typename TV, typename TX>
+void precond(const BMat<TV> &S1, const BMat<TV> &S2, const std::vector<int32_t> &cptr, const double *d1,
+             const double *d2, const TX *p, TX *y, TX *z) {
+    const int nc = (int)cptr.size() - 1;
+    if (d1 || !d2) {
+        sweep(S1, cptr, true, d1, p, y);
+        sweep(S2, cptr, false, d2, y, z);
+        return;
+    }
+    const std::vector<int32_t> head(cptr.begin(), cptr.end() - 1);   // colours 0..C-2
+    sweep(S1, head, true, (const double *)nullptr, p, y);
+    const int bs = S1.bs, bb = bs * bs;
+#pragma omp parallel for schedule(static)
+    for (int i = cptr[nc - 1]; i < cptr[nc]; ++i) {
+        double acc[4], t[4];
+        row_product(S1, i, y, acc);
+        for (int r = 0; r < bs; ++r) t[r] = (double)p[(size_t)i * bs + r] - acc[r];
+        const double *D = d2 + (size_t)i * bb;
+        for (int r = 0; r < bs; ++r) {
+            double v = 0.0;
+            for (int c = 0; c < bs; ++c) v += D[r * bs + c] * t[c];
+            z[(size_t)i * bs + r] = (TX)v;
+        }
+    }
+    sweep(S2, head, false, d2, y, z);
+}
+
 // Gauss-Jordan inverse with partial pivoting (first maximal |pivot|)
 bool invert(int bs, const double *a, double *inv) {
     double m[4][4], v[4][4];
@@ -271,8 +304,7 @@ RunState bicgstab(const OpSet<TV> &op, int64_t N, const double *rhs, TX *x, bool
     double alpha = 0, omega = 0, beta = 0;
     for (;;) {
         // steps 2-3: phat = M^-1 p
-        sweep(*op.S1, *op.cptr, true, op.d1, p.data(), y.data());
-        sweep(*op.S2, *op.cptr, false, op.d2, y.data(), phat.data());
+        precond(*op.S1, *op.S2, *op.cptr, op.d1, op.d2, p.data(), y.data(), phat.data());
         spmv(*op.A, phat.data(), v.data());                 // step 4
         const double vr = dot(N, v.data(), rt.data());       // step 5
         if (vr == 0.0 || !finite(vr)) {
@@ -292,8 +324,7 @@ RunState bicgstab(const OpSet<TV> &op, int64_t N, const double *rhs, TX *x, bool
         if (!(ns > thr)) { st.converged = true; return st; }
         if ((st.h >> 1) == maxit) { st.max_iter = true; return st; }
         st.h += 1;
-        sweep(*op.S1, *op.cptr, true, op.d1, s.data(), y.data());   // shat = M^-1 s
-        sweep(*op.S2, *op.cptr, false, op.d2, y.data(), shat.data());
+        precond(*op.S1, *op.S2, *op.cptr, op.d1, op.d2, s.data(), y.data(), shat.data());   // shat = M^-1 s
         spmv(*op.A, shat.data(), t.data());                  // step 9
         const double ts = dot(N, t.data(), s.data());        // step 10
         const double tt = dot(N, t.data(), t.data());
@@ -606,11 +637,9 @@ static void precond32(orc_handle h, const TX *p, TX *z, bool transpose) {
     std::vector<TX> y((size_t)h->n * h->bs);
     if (transpose) {
         ensure_adj32(h);
-        sweep(h->Ut32, h->cptr, true, h->dinvT.data(), p, y.data());
-        sweep(h->Lt32, h->cptr, false, (const double *)nullptr, y.data(), z);
+        precond(h->Ut32, h->Lt32, h->cptr, h->dinvT.data(), (const double *)nullptr, p, y.data(), z);
     } else {
-        sweep(h->L32, h->cptr, true, (const double *)nullptr, p, y.data());
-        sweep(h->U32, h->cptr, false, h->dinv.data(), y.data(), z);
+        precond(h->L32, h->U32, h->cptr, (const double *)nullptr, h->dinv.data(), p, y.data(), z);
     }
 }
 }  // extern "C++"
@@ -631,11 +660,9 @@ int orc_precond_perm(orc_handle h, const double *p, double *z, int32_t transpose
     std::vector<double> y((size_t)h->n * h->bs);
     if (transpose) {
         ensure_adj(h);
-        sweep(h->Ut, h->cptr, true, h->dinvT.data(), p, y.data());
-        sweep(h->Lt, h->cptr, false, (const double *)nullptr, y.data(), z);
+        precond(h->Ut, h->Lt, h->cptr, h->dinvT.data(), (const double *)nullptr, p, y.data(), z);
     } else {
-        sweep(h->L, h->cptr, true, (const double *)nullptr, p, y.data());
-        sweep(h->U, h->cptr, false, h->dinv.data(), y.data(), z);
+        precond(h->L, h->U, h->cptr, (const double *)nullptr, h->dinv.data(), p, y.data(), z);
     }
     return 0;
 }
