@@ -52,8 +52,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--mode", choices=["full", "kernels"], default="full",
-                    help="kernels: only repeated SpMV launches (FP64 and FP32 values) and a "
-                         "reference copy of known size, for rocprofv3 counter runs")
+                    help="kernels: only repeated SpMV launches and preconditioner applications "
+                         "(FP64 and FP32 values) and a reference copy of known size, for rocprofv3 "
+                         "counter runs")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the adjoint (64x64x32) and 4x4 (256x256x64) blocks")
@@ -453,6 +454,11 @@ def kernels_only(args, g, s, torch, dev, fa):
         by = g.algorithmic_bytes(fa.KIND_SPMV, fp32)
         res["spmv_fp32" if fp32 else "spmv_fp64"] = {"avg_ms": ms, "algorithmic_bytes": by,
                                                      "GBs": by / (ms * 1e-3) / 1e9}
+    for fp32 in (False, True):
+        ms = g.time_precond(args.reps, fp32)
+        by = g.algorithmic_bytes(fa.KIND_LOWER, fp32) + g.algorithmic_bytes(fa.KIND_UPPER, fp32)
+        res["precond_fp32" if fp32 else "precond_fp64"] = {"avg_ms": ms, "algorithmic_bytes": by,
+                                                           "GBs": by / (ms * 1e-3) / 1e9}
     # reference copy of known size (64 Mi doubles = 512 MiB read + 512 MiB written)
     src = torch.ones(1 << 26, dtype=torch.float64, device=dev)
     dst = torch.empty_like(src)

@@ -314,20 +314,26 @@ __device__ inline void block_partials(double (&v)[NRED], const RedArgs &ra, int 
 }
 
 constexpr int kFinThreads = 1024;
+// Loads in flight per thread: 16 (two passes cover 32,768 partials), or 32
+// for larger partial counts (the 4x4 256x256x64 grid's SpMV writes 95,044:
+// three passes instead of six); 16 running sums per thread either way, the
+// u-th load of a pass added to sum u mod 16.
+constexpr int kFinUnroll = 16, kFinUnrollWide = 32;
+constexpr int kFinWideFrom = 2 * kFinThreads * kFinUnroll + 1;
 
-template <int NRED>
+template <int NRED, int U>
 __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts) {
-    constexpr int U = 16;  // independent loads in flight per thread
     __shared__ double sred[kFinThreads / 64][2];
     if (ra.st->done) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int A = kFinUnroll;
     double acc[NRED];
 #pragma unroll
     for (int j = 0; j < NRED; ++j) {
         const double *pp = ra.partials + (size_t)j * ra.cap;
-        double a8[U];
+        double a8[A];
 #pragma unroll
-        for (int u = 0; u < U; ++u) a8[u] = 0.0;
+        for (int u = 0; u < A; ++u) a8[u] = 0.0;
         for (int i0 = 0; i0 < nparts; i0 += kFinThreads * U) {
             double v[U];
 #pragma unroll
@@ -336,11 +342,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts
                 v[u] = i < nparts ? pp[i] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) a8[u] += v[u];
+            for (int u = 0; u < U; ++u) a8[u % A] += v[u];
         }
         double t = a8[0];
 #pragma unroll
-        for (int u = 1; u < U; ++u) t += a8[u];
+        for (int u = 1; u < A; ++u) t += a8[u];
         acc[j] = wave_sum(t);
         if (lane == 0) sred[wave][j] = acc[j];
     }
