@@ -438,9 +438,9 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
 
 // Launches the streaming kernel, one workgroup per tile; returns the number
 // of reduction partials written.
-template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows, bool ZX = false>
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS = kTileRows, bool ZX = false, bool PLAIN = false>
 int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
-    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX, ROWS, ZX>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
+    hipLaunchKernelGGL((k_stream<BS, MODE, RED, TV, TX, ROWS, ZX, PLAIN>), dim3(a.ntiles), dim3(64 * BS), 0, h.stream, a);
     check_launch();
     return a.ntiles;
 }
@@ -503,8 +503,18 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     if (prof_kind >= 0) prof_begin(h, prof_kind);
     int nparts = 0;
     if constexpr (MODE != SM_SPMV && RED == SR_NONE) {
-        if (tall) nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRowsTall>(h, a);
-        else nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, ZX>(h, a);
+        bool done = false;
+        if constexpr (plain_small<TV>() && !ZX) {
+            if (a.ntiles < kPlainBelow) {
+                if (tall) nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRowsTall, false, true>(h, a);
+                else nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, false, true>(h, a);
+                done = true;
+            }
+        }
+        if (!done) {
+            if (tall) nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRowsTall>(h, a);
+            else nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, ZX>(h, a);
+        }
     } else {
         nparts = stream_dispatch<BS, MODE, RED, TV, TX, kTileRows, ZX>(h, a);
     }
