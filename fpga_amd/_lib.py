@@ -107,6 +107,7 @@ SIGNATURES = {
     "bda_precond_apply_perm": (i32, [vp, vp, vp, i32]),
     "bda_spmv_perm_ex": (i32, [vp, i32, vp, vp, i32, vp, i32, P_f64]),
     "bda_precond_apply_perm_ex": (i32, [vp, i32, vp, vp, i32]),
+    "bda_precond_launches": (i32, [vp, i32, P_i32]),
     "bda_spmv_dev": (i32, [vp, i32, vp, vp, i32, vp, i32, P_f64]),
     "bda_spmv_dev_noalloc": (i32, [vp, i32, vp, vp, i32, vp, i32, P_f64]),
     "bda_factor_sizes": (i32, [vp, P_i32, P_i32]),

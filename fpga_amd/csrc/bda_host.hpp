@@ -22,6 +22,10 @@ constexpr int kTileRowsTall = 128;
 // The merged launch around the last colour handles at most this many rows
 // with blocks in colour C-2 of a second sweep (one per thread group of BS).
 constexpr int kTailRows = 64;
+// The extended merged launch also handles at most kTailRows rows of colour
+// C-3; every row it computes from a packed list carries at most this many
+// second-sweep blocks.
+constexpr int kTail3Blocks = 4;
 inline int tile_cap(int bs, bool wide = false) { return (bs == 3 ? 168 : 128) * (wide ? 2 : 1); }
 
 struct Bcsr {

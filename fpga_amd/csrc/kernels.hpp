@@ -391,7 +391,12 @@ __global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const doub
 //
 // Summation order per scalar row r of block row i:
 //   sum = 0;  for blocks j in column order: sum += ((a_r0 x_0 + a_r1 x_1) + a_r2 x_2)
-enum StreamMode : int { SM_SPMV = 0, SM_SUB = 1, SM_SUB_DINV = 2 };
+//
+// SM_SUB_Z is SM_SUB for the first sweep's colour C-2 when the extended
+// merged tail (k_tail3) follows: rows without blocks in the second sweep
+// also get z = D^-1 y there, computed from y as stored (rounded to the
+// vector type), which is what the second sweep would compute for them.
+enum StreamMode : int { SM_SPMV = 0, SM_SUB = 1, SM_SUB_DINV = 2, SM_SUB_Z = 3 };
 
 template <typename TV, typename TX>
 struct StreamArgs {
@@ -408,9 +413,11 @@ struct StreamArgs {
     TX *out;
     TX *out2;           // optional copies of out (initial residual -> r, p)
     TX *out3;
-    const double *dinv; // SUB_DINV: out = dinv * (rhs - A x)
+    const double *dinv; // SUB_DINV: out = dinv * (rhs - A x); SUB_Z: the second sweep's inverse
     const TX *w0;       // SR_W0 reductions: sum out * w0
     RedArgs ra;
+    const int32_t *rowptr2 = nullptr;  // SUB_Z: the second sweep's row pointers
+    TX *zout = nullptr;                // SUB_Z: z of the rows without second-sweep blocks
 };
 
 template <typename TV, typename TX>
@@ -540,7 +547,7 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     __shared__ TX sx[CAP * BS];
     __shared__ int32_t scol[CJ * NT];
     __shared__ int32_t srow[NT];
-    __shared__ double tv[MODE == SM_SUB_DINV ? RS * NT : 1];
+    __shared__ double tv[(MODE == SM_SUB_DINV || MODE == SM_SUB_Z) ? RS * NT : 1];
 
     static_assert(!KEEP || ROWS == 64, "kept outputs: one row per thread");
     const int r0 = d.x, r1 = d.z, b0 = d.y;
@@ -574,6 +581,7 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     size_t gi[RS];
     double rhs_v[RS], w_v[RS];
     double dv[RS][BS];
+    bool zrow[RS];
 #pragma unroll
     for (int s = 0; s < RS; ++s) {
         kk[s] = k0 + 64 * s;
@@ -581,12 +589,17 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
         gi[s] = (size_t)(r0 + min(kk[s], nrow - 1)) * BS + r;
         rhs_v[s] = 0.0;
         w_v[s] = 0.0;
+        zrow[s] = false;
         if constexpr (MODE != SM_SPMV) rhs_v[s] = (double)a.rhs[gi[s]];
         if constexpr ((RED & SR_W0) != 0) w_v[s] = (double)a.w0[gi[s]];
-        if constexpr (MODE == SM_SUB_DINV) {
+        if constexpr (MODE == SM_SUB_DINV || MODE == SM_SUB_Z) {
             const double *D = a.dinv + gi[s] * BS;
 #pragma unroll
             for (int c = 0; c < BS; ++c) dv[s][c] = D[c];
+        }
+        if constexpr (MODE == SM_SUB_Z) {
+            const int i = r0 + min(kk[s], nrow - 1);
+            zrow[s] = a.rowptr2[i + 1] == a.rowptr2[i];
         }
     }
 
@@ -660,6 +673,22 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
 #pragma unroll
                 for (int c = 0; c < BS; ++c) z += dv[s][c] * tv[s * NT + k0 * BS + c];
                 res[s] = z;
+            }
+        }
+    }
+    if constexpr (MODE == SM_SUB_Z) {
+        // z = D^-1 (y - 0) from the stored y, with the second sweep's
+        // arithmetic (y - 0.0 is y exactly, signed zeros included)
+#pragma unroll
+        for (int s = 0; s < RS; ++s) tv[s * NT + t] = (double)from_double<TX>(res[s]);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < RS; ++s) {
+            if (has_row[s] && zrow[s]) {
+                double z = 0.0;
+#pragma unroll
+                for (int c = 0; c < BS; ++c) z += dv[s][c] * tv[s * NT + k0 * BS + c];
+                a.zout[gi[s]] = from_double<TX>(z);
             }
         }
     }
@@ -843,6 +872,145 @@ __global__ __launch_bounds__(64 * BS) void k_tail(StreamArgs<TV, TX> a, TailArgs
                 }
                 b.out[(size_t)i * BS + rr] = from_double<TX>(o);
             }
+        }
+    }
+}
+
+// The extended merged tail: k_tail's work on colours C-1 and C-2 and the
+// second sweep's colour C-3 in one launch, for a forward preconditioner
+// whose first sweep has a unit diagonal.  The first sweep's launch of colour
+// C-2 (SM_SUB_Z) has already written z for the rows of colour C-2 without
+// second-sweep blocks.  Of colour C-3, the rows whose blocks reach colour
+// C-1 or a colour C-2 row with blocks ("special" rows, at most 64) depend
+// on this launch's own results; the others only on z written before it.
+//  - workgroup 0: the first sweep's tile of colour C-1 (z kept in LDS), the
+//    colour C-2 rows with blocks (z kept in LDS), then the special rows of
+//    colour C-3;
+//  - the other workgroups: tiles of the other colour C-3 rows, as k_stream.
+// Workgroup 0 loads the operands of both row sets (packed lists of at most
+// kTail3Blocks blocks per row, with the source of each block's z) before the
+// tile's own loads, so that its chain stays at the tile's three memory
+// round trips.  Every row is computed with k_stream's arithmetic, in the
+// same order.
+template <typename TV, typename TX>
+struct Tail3Args {
+    const TV *val;           // second-sweep values
+    const TX *rhs2;          // y of colour C-2
+    const TX *rhs3;          // y of colour C-3 (p where the first sweep skipped colour 0)
+    TX *out;                 // z
+    const double *dinv;
+    int32_t n2, n3;          // colour C-2 rows with blocks; special rows of colour C-3
+    const int32_t *row;      // those rows: row[k] (k < n2), row[64 + k] (k < n3)
+    const int32_t *blk;      // their blocks, kTail3Blocks per row (the same layout), -1 past the last
+    const int32_t *src;      // z of the block's column: >= 0 that row, in memory; -1 - s: LDS slot s
+                             // (s < 64: row s of colour C-1; 64 + j: the j-th colour C-2 row)
+};
+
+template <int BS, typename TV, typename TX>
+struct Tail3Rows {
+    int32_t blk[kTail3Blocks], src[kTail3Blocks];
+    TV v[kTail3Blocks][BS];
+    TX zg[kTail3Blocks][BS];
+    double rhs, d[BS];
+};
+
+// operands of packed row k (slot base + k) for thread component r; GZ: some
+// blocks may take z from memory
+template <int BS, bool GZ, typename TV, typename TX>
+__device__ inline void tail3_load(const Tail3Args<TV, TX> &b, const TX *rhs, int base, int k, int n, int r,
+                                  Tail3Rows<BS, TV, TX> &o) {
+    constexpr int BB = BS * BS;
+    const int kc = min(k, max(n - 1, 0));
+    const int i = n > 0 ? b.row[base + kc] : 0;
+#pragma unroll
+    for (int m = 0; m < kTail3Blocks; ++m) {
+        o.blk[m] = n > 0 ? b.blk[(base + kc) * kTail3Blocks + m] : -1;
+        o.src[m] = n > 0 ? b.src[(base + kc) * kTail3Blocks + m] : -1;
+    }
+    const size_t gi = (size_t)i * BS + r;
+    o.rhs = (double)rhs[gi];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) o.d[c] = b.dinv[gi * BS + c];
+#pragma unroll
+    for (int m = 0; m < kTail3Blocks; ++m) {
+        const TV *av = b.val + (size_t)max(o.blk[m], 0) * BB + r * BS;
+        const TX *zp = b.out + (size_t)max(o.src[m], 0) * BS;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+            o.v[m][c] = av[c];
+            o.zg[m][c] = GZ ? zp[c] : (TX)0;
+        }
+    }
+}
+
+// rhs - sum over the row's blocks, in column order, of the block products
+template <int BS, bool GZ, typename TV, typename TX>
+__device__ inline double tail3_row(const Tail3Rows<BS, TV, TX> &o, const TX *zk, const TX *zd) {
+    double sum = 0.0;
+#pragma unroll
+    for (int m = 0; m < kTail3Blocks; ++m) {
+        if (o.blk[m] < 0) break;
+        TX zc[BS];
+        const bool mem = GZ && o.src[m] >= 0;
+        const int sl = mem ? 0 : -1 - o.src[m];
+        const TX *zl = sl < 64 ? zk + sl * BS : zd + (sl - 64) * BS;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) zc[c] = mem ? o.zg[m][c] : zl[c];
+        double sm = prod<TV, TX>(o.v[m][0], zc[0]);
+        sm = sm + prod<TV, TX>(o.v[m][1], zc[1]);
+        sm = sm + prod<TV, TX>(o.v[m][2], zc[2]);
+        if constexpr (BS == 4) sm = sm + prod<TV, TX>(o.v[m][3], zc[3]);
+        sum += sm;
+    }
+    return o.rhs - sum;
+}
+
+template <int BS, int ROWSB, typename TV, typename TX>
+__global__ __launch_bounds__(64 * BS) void k_tail3(StreamArgs<TV, TX> a, Tail3Args<TV, TX> b,
+                                                   StreamArgs<TV, TX> c) {
+    constexpr int NT = 64 * BS;
+    if (a.ra.st->done) return;
+    const int t = threadIdx.x;
+    if (blockIdx.x > 0) {
+        const int4 d = c.tile[c.tile0 + xcd_tile(blockIdx.x - 1, c.ntiles)];
+        stream_tile<BS, SM_SUB_DINV, SR_NONE, TV, TX, ROWSB, false, plain_small<TV>()>(c, d, nullptr, 0);
+        return;
+    }
+    __shared__ TX zk[64 * BS];
+    __shared__ TX zd[64 * BS];
+    __shared__ double tvb[NT];
+    const int k = t / BS, r = t - k * BS;
+    Tail3Rows<BS, TV, TX> o2, o3;
+    tail3_load<BS, false, TV, TX>(b, b.rhs2, 0, k, b.n2, r, o2);
+    tail3_load<BS, true, TV, TX>(b, b.rhs3, 64, k, b.n3, r, o3);
+    const int4 d = a.tile[a.tile0];
+    stream_tile<BS, SM_SUB_DINV, SR_NONE, TV, TX, 64, true, plain_small<TV>()>(a, d, zk, 0);
+    __syncthreads();
+    // colour C-2 rows with blocks (only into colour C-1)
+    {
+        const bool has = k < b.n2;
+        tvb[t] = has ? tail3_row<BS, false, TV, TX>(o2, zk, zd) : 0.0;
+        __syncthreads();
+        if (has) {
+            double z = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < BS; ++cc) z += o2.d[cc] * tvb[k * BS + cc];
+            const TX zo = from_double<TX>(z);
+            b.out[(size_t)b.row[k] * BS + r] = zo;
+            zd[t] = zo;
+        }
+    }
+    __syncthreads();
+    // the special rows of colour C-3
+    {
+        const bool has = k < b.n3;
+        tvb[t] = has ? tail3_row<BS, true, TV, TX>(o3, zk, zd) : 0.0;
+        __syncthreads();
+        if (has) {
+            double z = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < BS; ++cc) z += o3.d[cc] * tvb[k * BS + cc];
+            b.out[(size_t)b.row[64 + k] * BS + r] = from_double<TX>(z);
         }
     }
 }

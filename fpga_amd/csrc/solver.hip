@@ -77,6 +77,17 @@ struct DevMat {
     DBuf<int32_t> tail_dep;
     int tail_ndep = -1;
     int tail_row0 = 0, tail_nrows = 0;
+    // the extended merged tail (k_tail3), when also at most kTailRows rows of
+    // colour C-3 have blocks into colour C-1 or into a row of tail_dep, and
+    // every row it packs has at most kTail3Blocks blocks; else tail3_n = -1.
+    // tail3_row / tail3_blk / tail3_src: the packed rows (k_tail3's
+    // Tail3Args); tail3_n: the special rows of colour C-3; tail3_tile: tiles
+    // of the other colour C-3 rows, tall when that saves a quarter of the
+    // workgroups.
+    DBuf<int32_t> tail3_row, tail3_blk, tail3_src;
+    DBuf<int4> tail3_tile;
+    int tail3_n = -1, tail3_ntiles = 0;
+    bool tail3_tall = false;
 
     // slices per layer in the launch order, one per XCD (measured on a
     // 128x128x64 grid: 4 slices no better, 16 and 32 slower, FP64 iteration
@@ -193,8 +204,114 @@ struct DevMat {
         upload_tiles(p, p.tile_row_tall, p.colour_tile_tall, orig_row, span, tile_tall, slot_tall, nullptr, nullptr,
                      s);
     }
+    // Launch-order key of a tile whose first row is r (upload_tiles).
+    static int64_t tile_key(const std::vector<int32_t> &o, int64_t n, int64_t L, int r) {
+        const int64_t orow = o[r];
+        const int64_t slice = (L >= kLayerSlices && 2 * L <= n) ? ((orow % L) * kLayerSlices) / L : 0;
+        return slice * (n + 1) + orow;
+    }
+
+    // Tiles over the row ranges runs (as build_tiles does for a colour), in
+    // launch order; false when a row has more blocks than a tile holds.
+    static bool tiles_for_runs(const MatPattern &p, const std::vector<std::pair<int, int>> &runs, int cap,
+                               int max_rows, const std::vector<int32_t> &o, int64_t L, std::vector<int4> &out) {
+        std::vector<std::pair<int64_t, int4>> t;
+        for (const auto &run : runs) {
+            int r = run.first;
+            while (r < run.second) {
+                const int first = r;
+                int rows = 0, blocks = 0;
+                while (r < run.second && rows < max_rows) {
+                    const int nb = p.rowptr[r + 1] - p.rowptr[r];
+                    if (nb > cap) return false;
+                    if (blocks + nb > cap) break;
+                    blocks += nb;
+                    ++rows;
+                    ++r;
+                }
+                t.push_back({tile_key(o, p.n, L, first), make_int4(first, p.rowptr[first], r, p.rowptr[r])});
+            }
+        }
+        std::stable_sort(t.begin(), t.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+        out.resize(t.size());
+        for (size_t i = 0; i < t.size(); ++i) out[i] = t[i].second;
+        return true;
+    }
+
+    // The rows k_tail3 computes in its first workgroup (the colour C-2 rows
+    // of tail_dep, then the special rows of colour C-3), packed with their
+    // blocks and each block's z source, and tiles of the other colour C-3 rows.
+    void set_tail3(const MatPattern &p, const std::vector<int32_t> &colour_ptr, const std::vector<int32_t> &o,
+                   int64_t span, int bs, hipStream_t s) {
+        tail3_n = -1;
+        const int nc = (int)colour_ptr.size() - 1;
+        if (tail_ndep < 0 || nc < 3) return;
+        const int c3_0 = colour_ptr[nc - 3], c3_1 = colour_ptr[nc - 2];
+        const int last0 = colour_ptr[nc - 1];
+        std::vector<int32_t> dep(tail_ndep);
+        if (tail_ndep > 0) HIP_CHECK(hipMemcpy(dep.data(), tail_dep.p, tail_ndep * 4, hipMemcpyDeviceToHost));
+        std::vector<int32_t> dslot(tail_nrows, -1);
+        for (int j = 0; j < tail_ndep; ++j) dslot[dep[j] - tail_row0] = j;
+        constexpr int MB = kTail3Blocks;
+        std::vector<int32_t> row(2 * kTailRows, 0), blk(2 * kTailRows * MB, -1), src(2 * kTailRows * MB, -1);
+        // one packed row; false when it has too many blocks or a block that
+        // points before colour C-2
+        auto pack = [&](int slot, int r) {
+            if (p.rowptr[r + 1] - p.rowptr[r] > MB) return false;
+            row[slot] = r;
+            int m = 0;
+            for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q, ++m) {
+                const int col = p.colidx[q];
+                if (col < tail_row0) return false;
+                blk[slot * MB + m] = q;
+                if (col >= last0) src[slot * MB + m] = -1 - (col - last0);
+                else if (dslot[col - tail_row0] >= 0) src[slot * MB + m] = -1 - (kTailRows + dslot[col - tail_row0]);
+                else src[slot * MB + m] = col;
+            }
+            return true;
+        };
+        for (int j = 0; j < tail_ndep; ++j)
+            if (!pack(j, dep[j])) return;
+        int n3 = 0;
+        std::vector<std::pair<int, int>> runs;
+        int run0 = c3_0;
+        for (int r = c3_0; r < c3_1; ++r) {
+            bool special = false;
+            for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q) {
+                const int col = p.colidx[q];
+                if (col < tail_row0) return;  // second-sweep blocks only point to later colours
+                if (col >= last0 || dslot[col - tail_row0] >= 0) special = true;
+            }
+            if (!special) continue;
+            if (n3 == kTailRows || !pack(kTailRows + n3, r)) return;
+            ++n3;
+            if (r > run0) runs.push_back({run0, r});
+            run0 = r + 1;
+        }
+        if (c3_1 > run0) runs.push_back({run0, c3_1});
+        std::vector<int4> narrow, tall;
+        if (!tiles_for_runs(p, runs, tile_cap(bs), kTileRows, o, span, narrow) ||
+            !tiles_for_runs(p, runs, tile_cap(bs), kTileRowsTall, o, span, tall))
+            return;
+        tail3_tall = 4 * tall.size() <= 3 * narrow.size();
+        const std::vector<int4> &tl = tail3_tall ? tall : narrow;
+        tail3_ntiles = (int)tl.size();
+        tail3_tile.ensure(std::max<size_t>(tl.size(), 1));
+        if (!tl.empty())
+            HIP_CHECK(hipMemcpyAsync(tail3_tile.p, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        tail3_row.ensure(row.size());
+        tail3_blk.ensure(blk.size());
+        tail3_src.ensure(src.size());
+        HIP_CHECK(hipMemcpyAsync(tail3_row.p, row.data(), row.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(tail3_blk.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(tail3_src.p, src.data(), src.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        tail3_n = n3;
+    }
+
     void set_tail(const MatPattern &p, const std::vector<int32_t> &colour_ptr, hipStream_t s) {
         tail_ndep = -1;
+        tail3_n = -1;
         const int nc = (int)colour_ptr.size() - 1;
         if (nc < 3) return;
         tail_row0 = colour_ptr[nc - 2];
@@ -225,6 +342,9 @@ struct DevMat {
         tile_tall.release();
         for (DBuf<int32_t> *b : {&slot, &slot_sp, &slot_wide, &slot_wide_sp, &slot_tall}) b->release();
         tail_dep.release();
+        for (DBuf<int32_t> *b : {&tail3_row, &tail3_blk, &tail3_src}) b->release();
+        tail3_tile.release();
+        tail3_n = -1;
         src.release();
     }
 };
@@ -260,6 +380,10 @@ enum ProfKind { PK_SPMV = 0, PK_LOWER = 1, PK_UPPER = 2, PK_VEC = 3, PK_N = 4 };
 }  // namespace
 }  // namespace bda
 
+#ifndef BDA_TAIL3
+#define BDA_TAIL3 1
+#endif
+
 struct bda_handle_s {
     int bs = 0, n = 0;
     int64_t nnzb_max = 0;
@@ -284,6 +408,8 @@ struct bda_handle_s {
     bda::DBuf<float> At32, Ut32, Lt32;  // FP32 transposes (component access only)
     uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX, adj32_gen = UINT64_MAX;
     uint64_t values_gen = 0;  // changes with every upload, value update and factorisation
+    // the extended merged tail launch (k_tail3) where the colouring allows it
+    bool tail3 = BDA_TAIL3 != 0;
 
     // vectors
     bda::Krylov<double> k64;
@@ -453,7 +579,8 @@ int stream_dispatch(H &h, const StreamArgs<TV, TX> &a) {
 template <int BS, int MODE, int RED, typename TV, typename TX, bool ZX = false>
 void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x, const TX *rhs, TX *out,
                    const double *dinv, const TX *w0, RedArgs ra, bool use_state, TX *out2 = nullptr,
-                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1) {
+                   TX *out3 = nullptr, const TX *x_lo = nullptr, int split = 0, int prof_kind = -1,
+                   const int32_t *rowptr2 = nullptr, TX *zout = nullptr) {
     constexpr int nred = (RED & SR_W0 ? 1 : 0) + (RED & SR_SELF ? 1 : 0);
     StreamArgs<TV, TX> a;
     constexpr bool wide = wide_tiles<MODE>();
@@ -496,7 +623,10 @@ void launch_stream(H &h, const DevMat &m, const TV *val, int colour, const TX *x
     a.dinv = dinv;
     a.w0 = w0;
     a.ra = ra;
+    a.rowptr2 = rowptr2;
+    a.zout = zout;
     if (!use_state) a.ra.st = h.st.p + kIdleState;
+    if (MODE == SM_SUB_Z && (!rowptr2 || !zout || !dinv)) throw std::runtime_error("SUB_Z launch without z operands");
     if ((RED & SR_W0) && !w0) throw std::runtime_error("missing dot operand");
     if (nred > 0 && !use_state) throw std::runtime_error("a reduction needs the solver state");
     if (nred > 0 && a.ntiles > a.ra.cap) throw std::runtime_error("partials buffer too small");
@@ -611,6 +741,72 @@ void tail_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split
     else tail_dispatch<BS, SM_SUB, SM_SUB, TV, TX>(h, a, b);
 }
 
+template <int BS, int ROWSB, typename TV, typename TX>
+void tail3_dispatch(H &h, const StreamArgs<TV, TX> &a, const Tail3Args<TV, TX> &b, const StreamArgs<TV, TX> &c) {
+    hipLaunchKernelGGL((k_tail3<BS, ROWSB, TV, TX>), dim3(1 + c.ntiles), dim3(64 * BS), 0, h.stream, a, b, c);
+    check_launch();
+}
+
+// The extended merged tail (k_tail3) of a forward application whose first
+// sweep has a unit diagonal: the first sweep's colour C-1, the second
+// sweep's colours C-2 and C-3; rhs3 is y, or p when colour C-3 is colour 0
+// and the first sweep skipped it.
+template <int BS, typename TV, typename TX>
+void tail3_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int split, const TX *p, TX *z,
+                  const TX *rhs3, bool use_state) {
+    const int nc = h.an.ncolours;
+    const DevMat &m1 = *op.S1, &m2 = *op.S2;
+    if (op.s1_dinv || !op.s2_dinv || m2.tail3_n < 0 || m2.tail_ndep < 0 || nc < 3 ||
+        m1.colour_tile[nc] - m1.colour_tile[nc - 1] != 1 || m2.tail_row0 != h.an.colour_ptr[nc - 2])
+        throw std::runtime_error("extended tail launch does not fit the colouring");
+    StreamArgs<TV, TX> a;
+    std::memset(&a, 0, sizeof(a));
+    a.tile = m1.tile.p;
+    a.slot = m1.slot.p;
+    a.tile0 = m1.colour_tile[nc - 1];
+    a.ntiles = 1;
+    a.rowptr = m1.rowptr.p;
+    a.colidx = m1.colidx.p;
+    a.val = op.S1val;
+    a.x = y;
+    a.x_lo = lo ? lo : y;
+    a.split = lo ? split : 0;
+    a.rhs = p;
+    a.out = z;
+    a.dinv = op.dinv;
+    a.ra = red(h, 0, FIN_NONE);
+    if (!use_state) a.ra.st = h.st.p + kIdleState;
+    Tail3Args<TV, TX> b;
+    b.val = op.S2val;
+    b.rhs2 = y;
+    b.rhs3 = rhs3;
+    b.out = z;
+    b.dinv = op.dinv;
+    b.n2 = m2.tail_ndep;
+    b.n3 = m2.tail3_n;
+    b.row = m2.tail3_row.p;
+    b.blk = m2.tail3_blk.p;
+    b.src = m2.tail3_src.p;
+    StreamArgs<TV, TX> c;
+    std::memset(&c, 0, sizeof(c));
+    c.tile = m2.tail3_tile.p;
+    c.slot = nullptr;  // no reductions
+    c.tile0 = 0;
+    c.ntiles = m2.tail3_ntiles;
+    c.rowptr = m2.rowptr.p;
+    c.colidx = m2.colidx.p;
+    c.val = op.S2val;
+    c.x = z;
+    c.x_lo = z;
+    c.split = 0;
+    c.rhs = rhs3;
+    c.out = z;
+    c.dinv = op.dinv;
+    c.ra = a.ra;
+    if (m2.tail3_tall) tail3_dispatch<BS, kTileRowsTall, TV, TX>(h, a, b, c);
+    else tail3_dispatch<BS, kTileRows, TV, TX>(h, a, b, c);
+}
+
 // z = M^{-1} p through the two colour sweeps of op (first sweep colours
 // ascending into y, second sweep colours descending into z).  Two launches
 // are saved per application:
@@ -633,19 +829,45 @@ void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_stat
     const int split = skip0 ? h.an.colour_ptr[1] : 0;
     const bool merge = nc >= 3 && op.S2->tail_ndep >= 0 &&
                        op.S1->colour_tile[nc] - op.S1->colour_tile[nc - 1] == 1;
+    const bool merge3 = merge && h.tail3 && !op.s1_dinv && op.s2_dinv && op.S2->tail3_n >= 0;
     prof_begin(h, PK_LOWER);
-    for (int c = skip0 ? 1 : 0; c < nc - 1; ++c)
-        sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
+    for (int c = skip0 ? 1 : 0; c < nc - 1; ++c) {
+        if (merge3 && c == nc - 2)
+            launch_stream<BS, SM_SUB_Z, SR_NONE, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, op.dinv, nullptr,
+                                                         red(h, 0, FIN_NONE), use_state, nullptr, nullptr, lo, split,
+                                                         -1, op.S2->rowptr.p, z);
+        else
+            sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, c, op.s1_dinv, y, lo, split, p, y, op.dinv, use_state);
+    }
     if (!merge)
         sweep_launch<BS, TV, TX>(h, *op.S1, op.S1val, nc - 1, op.s1_dinv || op.s2_dinv, y, lo, split, p, z, op.dinv,
                                  use_state);
     prof_end(h);
     prof_begin(h, PK_UPPER);
-    if (merge) tail_launch<BS, TV, TX>(h, op, y, lo, split, p, z, use_state);
-    for (int c = merge ? nc - 3 : nc - 2; c >= 0; --c)
+    if (merge3) tail3_launch<BS, TV, TX>(h, op, y, lo, split, p, z, (nc - 3 == 0 && skip0) ? p : y, use_state);
+    else if (merge) tail_launch<BS, TV, TX>(h, op, y, lo, split, p, z, use_state);
+    for (int c = merge3 ? nc - 4 : merge ? nc - 3 : nc - 2; c >= 0; --c)
         sweep_launch<BS, TV, TX>(h, *op.S2, op.S2val, c, op.s2_dinv, z, nullptr, 0, (c == 0 && skip0) ? p : y, z,
                                  op.dinv, use_state);
     prof_end(h);
+}
+
+// Kernel launches of one application of op's preconditioner (as precond
+// issues them).
+template <typename TV>
+int precond_launches(const H &h, const OpSet<TV> &op) {
+    const int nc = h.an.ncolours;
+    const bool skip0 = !op.s1_dinv && nc > 1;
+    const bool merge = nc >= 3 && op.S2->tail_ndep >= 0 &&
+                       op.S1->colour_tile[nc] - op.S1->colour_tile[nc - 1] == 1;
+    const bool merge3 = merge && h.tail3 && !op.s1_dinv && op.s2_dinv && op.S2->tail3_n >= 0;
+    auto nonempty = [&](const DevMat &m, int c) { return m.colour_tile[c + 1] > m.colour_tile[c] ? 1 : 0; };
+    int k = 0;
+    for (int c = skip0 ? 1 : 0; c < nc - 1; ++c) k += nonempty(*op.S1, c);
+    if (!merge) k += nonempty(*op.S1, nc - 1);
+    if (merge || merge3) k += 1;
+    for (int c = merge3 ? nc - 4 : merge ? nc - 3 : nc - 2; c >= 0; --c) k += nonempty(*op.S2, c);
+    return k;
 }
 
 struct RunOut {
@@ -884,6 +1106,7 @@ void finish_analyse(H &h) {
     h.Ut.upload(an.Ut, h.stream, &an.perm, span);
     h.Lt.upload(an.Lt, h.stream, &an.perm, span);
     h.U.set_tail(an.U, an.colour_ptr, h.stream);
+    h.U.set_tail3(an.U, an.colour_ptr, an.perm, span, h.bs, h.stream);
     h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
     h.d_diag.ensure(h.n);
@@ -1804,6 +2027,14 @@ int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z
     return guarded(h, [&] {
         if (!p || !z) throw std::runtime_error("null vector");
         BS_CALL(h, precond_perm, *h, prec, p, z, transpose);
+    });
+}
+
+int bda_precond_launches(bda_handle h, int32_t transpose, int32_t *launches) {
+    return guarded(h, [&] {
+        require_analysed(*h);
+        if (!launches) throw std::runtime_error("null output");
+        *launches = transpose ? precond_launches(*h, op_adj64(*h)) : precond_launches(*h, op_fwd64(*h));
     });
 }
 

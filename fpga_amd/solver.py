@@ -361,6 +361,12 @@ class BicgstabSolver:
                     "bda_precond_apply_perm_ex")
         return z
 
+    def precond_launches(self, transpose: bool = False) -> int:
+        """Kernel launches of one preconditioner application."""
+        k = C.c_int32()
+        self._check(lib.bda_precond_launches(self._h, int(transpose), C.byref(k)), "bda_precond_launches")
+        return k.value
+
     def factors(self):
         nl, nu = C.c_int32(), C.c_int32()
         self._check(lib.bda_factor_sizes(self._h, C.byref(nl), C.byref(nu)), "bda_factor_sizes")

@@ -262,6 +262,13 @@ int bda_precond_apply_perm(bda_handle h, const double *p, double *z, int32_t tra
 int bda_spmv_perm_ex(bda_handle h, int32_t prec, const void *x, void *y, int32_t transpose, const void *w,
                      int32_t red, double dots[2]);
 int bda_precond_apply_perm_ex(bda_handle h, int32_t prec, const void *p, void *z, int32_t transpose);
+/* Kernel launches one application of the preconditioner makes (transpose
+ * != 0: of the adjoint's), after bda_analyse: one per colour and sweep, less
+ * the launches the colouring lets the library merge.  No counterpart in the
+ * reference, whose sweeps run colour by colour inside one kernel run
+ * (rtl/bicgstab_solver_2r-3r3w/int_read_unit.vhd:287-302); for callers that
+ * budget launch latency. */
+int bda_precond_launches(bda_handle h, int32_t transpose, int32_t *launches);
 /* y = A x (transpose != 0: A^T x) on the caller's device vectors (float for
  * prec 1, double otherwise) in the colour-ordered numbering, with the fused
  * reductions of bda_spmv_perm_ex; for callers that drive their own Krylov
