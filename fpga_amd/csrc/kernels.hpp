@@ -361,6 +361,45 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(RedArgs ra, int nparts
     }
 }
 
+// k_finalize for at most kFinThreads * U partials (U < kFinUnroll): one pass
+// that issues the loads of all NRED reductions together (k_finalize sums
+// them one reduction after the other), with the same sums: k_finalize's
+// running sums past the first U stay +0.0, and adding them changes the
+// total only as one final + 0.0 does.
+template <int NRED, int U>
+__global__ __launch_bounds__(kFinThreads) void k_finalize_one_pass(RedArgs ra, int nparts) {
+    static_assert(U < kFinUnroll, "one pass of fewer loads than k_finalize's running sums");
+    __shared__ double sred[kFinThreads / 64][2];
+    if (ra.st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[NRED][U];
+#pragma unroll
+    for (int j = 0; j < NRED; ++j)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int i = u * kFinThreads + threadIdx.x;
+            v[j][u] = i < nparts ? ra.partials[(size_t)j * ra.cap + i] : 0.0;
+        }
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+        double t = 0.0 + v[j][0];
+#pragma unroll
+        for (int u = 1; u < U; ++u) t += 0.0 + v[j][u];
+        t += 0.0;
+        const double acc = wave_sum(t);
+        if (lane == 0) sred[wave][j] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[2] = {0.0, 0.0};
+        for (int j = 0; j < NRED; ++j) {
+            t[j] = sred[0][j];
+            for (int w = 1; w < kFinThreads / 64; ++w) t[j] += sred[w][j];
+        }
+        finalize(ra.st, ra.fin, t[0], t[1]);
+    }
+}
+
 // dst = src (n doubles); *nonfinite = 1 when a value is infinite or NaN.
 // Every writer stores the same value.
 __global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const double *src, double *dst,

@@ -552,7 +552,10 @@ RedArgs red(H &h, int state_idx, int fin) {
 // grid-wide sum of the partials of the kernel just launched + scalar epilogue
 void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
     const bool wide = nparts >= kFinWideFrom;
-    if (nred == 1) {
+    if (nparts <= 2 * kFinThreads) {
+        if (nred == 1) hipLaunchKernelGGL((k_finalize_one_pass<1, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+        else hipLaunchKernelGGL((k_finalize_one_pass<2, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+    } else if (nred == 1) {
         if (wide) hipLaunchKernelGGL((k_finalize<1, kFinUnrollWide>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
         else hipLaunchKernelGGL((k_finalize<1, kFinUnroll>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
     } else {
