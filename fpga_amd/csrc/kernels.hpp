@@ -400,6 +400,49 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_one_pass(RedArgs ra, i
     }
 }
 
+// Above kFinTwoStageFrom partials (the 4x4 256x256x64 grid's SpMV writes
+// 95,044) one workgroup reads them at about 55 GB/s; a first stage of
+// kFinGroups workgroups sums contiguous chunks instead (each chunk in a fixed
+// order), into the last kFinGroups slots of each reduction's row of the
+// partials buffer, and k_finalize_one_pass adds those up.
+constexpr int kFinGroups = 256, kFinGroupThreads = 256, kFinGroupUnroll = 4;
+constexpr int kFinTwoStageFrom = 2 * kFinThreads * kFinUnrollWide + 1;
+
+template <int NRED>
+__global__ __launch_bounds__(kFinGroupThreads) void k_finalize_stage1(RedArgs ra, int nparts) {
+    __shared__ double sred[kFinGroupThreads / 64][2];
+    if (ra.st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int chunk = (nparts + kFinGroups - 1) / kFinGroups;
+    const int i0 = blockIdx.x * chunk, i1 = min(i0 + chunk, nparts);
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+        const double *pp = ra.partials + (size_t)j * ra.cap;
+        double t = 0.0;
+        for (int i = i0 + threadIdx.x; i < i1; i += kFinGroupThreads * kFinGroupUnroll) {
+            double v[kFinGroupUnroll];
+#pragma unroll
+            for (int u = 0; u < kFinGroupUnroll; ++u) {
+                const int k = i + u * kFinGroupThreads;
+                v[u] = k < i1 ? pp[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kFinGroupUnroll; ++u) t += v[u];
+        }
+        t = wave_sum(t);
+        if (lane == 0) sred[wave][j] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NRED; ++j) {
+            double t = sred[0][j];
+            for (int w = 1; w < kFinGroupThreads / 64; ++w) t += sred[w][j];
+            ra.partials[(size_t)j * ra.cap + ra.cap - kFinGroups + blockIdx.x] = t;
+        }
+    }
+}
+
 // dst = src (n doubles); *nonfinite = 1 when a value is infinite or NaN.
 // Every writer stores the same value.
 __global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const double *src, double *dst,

@@ -552,7 +552,16 @@ RedArgs red(H &h, int state_idx, int fin) {
 // grid-wide sum of the partials of the kernel just launched + scalar epilogue
 void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
     const bool wide = nparts >= kFinWideFrom;
-    if (nparts <= 2 * kFinThreads) {
+    if (nparts >= kFinTwoStageFrom) {
+        if (nparts > ra.cap - kFinGroups) throw std::runtime_error("partials buffer too small for the two-stage sum");
+        if (nred == 1) hipLaunchKernelGGL((k_finalize_stage1<1>), dim3(kFinGroups), dim3(kFinGroupThreads), 0, h.stream, ra, nparts);
+        else hipLaunchKernelGGL((k_finalize_stage1<2>), dim3(kFinGroups), dim3(kFinGroupThreads), 0, h.stream, ra, nparts);
+        check_launch();
+        RedArgs r2 = ra;
+        r2.partials = ra.partials + (ra.cap - kFinGroups);
+        if (nred == 1) hipLaunchKernelGGL((k_finalize_one_pass<1, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, r2, kFinGroups);
+        else hipLaunchKernelGGL((k_finalize_one_pass<2, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, r2, kFinGroups);
+    } else if (nparts <= 2 * kFinThreads) {
         if (nred == 1) hipLaunchKernelGGL((k_finalize_one_pass<1, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
         else hipLaunchKernelGGL((k_finalize_one_pass<2, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
     } else if (nred == 1) {
@@ -1131,8 +1140,9 @@ void finish_analyse(H &h) {
     h.DinvT.ensure((size_t)h.n * bb);
     h.d_bad.ensure(1);
     h.d_nonfinite.ensure(1);
-    // reduction partials: one slot pair per workgroup of the largest reducing launch
-    const int cap = std::max({h.A.ntiles, h.At.ntiles, vec_blocks(h.N()), 1});
+    // reduction partials: one slot pair per workgroup of the largest reducing
+    // launch, and room for the first stage of a two-stage sum
+    const int cap = std::max({h.A.ntiles, h.At.ntiles, vec_blocks(h.N()), 1}) + kFinGroups;
     h.partial_cap = cap;
     h.partials.ensure((size_t)2 * cap);
     h.k64.ensure(h.Npad());
