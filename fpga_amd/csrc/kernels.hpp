@@ -500,6 +500,7 @@ struct StreamArgs {
     RedArgs ra;
     const int32_t *rowptr2 = nullptr;  // SUB_Z: the second sweep's row pointers
     TX *zout = nullptr;                // SUB_Z: z of the rows without second-sweep blocks
+    const int32_t *rowmap = nullptr;   // dependency tiles (k_lower2): matrix row of each packed row
 };
 
 template <typename TV, typename TX>
@@ -601,13 +602,39 @@ template <int RED> struct NRedOf { static constexpr int value = (RED & 1) + ((RE
 // ROWS = 128 (the tall tiling, used by sweep colours with few blocks per row)
 // gives every thread two rows of the epilogue, 64 rows apart.
 //
+template <typename TX, int BS, int CAP>
+struct TileLds {
+    double part[CAP * BS];
+    TX sx[CAP * BS];
+    int32_t scol[(CAP + 64 * BS - 1) / (64 * BS) * 64 * BS];
+    int32_t srow[64 * BS];
+};
+template <typename TX, int BS, int CAP>
+__device__ inline TileLds<TX, BS, CAP> &tile_lds() {
+    __shared__ TileLds<TX, BS, CAP> l;
+    return l;
+}
+template <int N>
+__device__ inline double *tile_tv() {
+    __shared__ double tv[N];
+    return tv;
+}
+
 // stream_tile is the work of one workgroup on the tile with descriptor d;
 // with KEEP it also leaves the tile's outputs (rounded to TX, (row - first
 // row) * BS + component) in keep[], an LDS array of 64 * BS entries.  Its
 // fused reductions write partial-sum slot `slot`.
-template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS, bool KEEP, bool PLAIN = false>
+//
+// V selects the two tile forms of the merged lower-sweep launch (k_lower2):
+// 1, a dependency tile: packed copies of matrix rows (a.rowmap gives each
+// packed row's matrix row, for its right-hand side); its outputs only go to
+// keep[]; 2, a tile whose column indices below zero name a row of the
+// dependency tile: -1 - c is the row's place there, and its x comes from
+// ldsx[] (that tile's keep[]) instead of memory.
+template <int BS, int MODE, int RED, typename TV, typename TX, int ROWS, bool KEEP, bool PLAIN = false, int V = 0>
 __device__ __attribute__((always_inline)) inline void stream_tile(const StreamArgs<TV, TX> &a, const int4 d, TX *keep,
-                                                                   int slot) {
+                                                                   int slot, const TX *ldsx = nullptr) {
+    static_assert(V != 1 || (KEEP && MODE == SM_SUB && RED == SR_NONE), "dependency tiles keep plain sweep rows");
     constexpr int BB = BS * BS;
     constexpr int NT = 64 * BS;            // threads per workgroup
     constexpr int NW = BS;                 // waves per workgroup
@@ -624,12 +651,15 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
 
     // LDS arrays are sized to whole multiples of the workgroup so that their
     // fills are unconditional (a conditional store lets the compiler sink the
-    // feeding load next to it, where it is waited for at once)
-    __shared__ double part[CAP * BS];
-    __shared__ TX sx[CAP * BS];
-    __shared__ int32_t scol[CJ * NT];
-    __shared__ int32_t srow[NT];
-    __shared__ double tv[(MODE == SM_SUB_DINV || MODE == SM_SUB_Z) ? RS * NT : 1];
+    // feeding load next to it, where it is waited for at once).  They belong
+    // to the tile shape, not to this instance, so that the instances one
+    // kernel runs one after the other share them.
+    TileLds<TX, BS, CAP> &lds = tile_lds<TX, BS, CAP>();
+    double *part = lds.part;
+    TX *sx = lds.sx;
+    int32_t *scol = lds.scol;
+    int32_t *srow = lds.srow;
+    double *tv = tile_tv<(MODE == SM_SUB_DINV || MODE == SM_SUB_Z) ? RS * NT : 1>();
 
     static_assert(!KEEP || ROWS == 64, "kept outputs: one row per thread");
     const int r0 = d.x, r1 = d.z, b0 = d.y;
@@ -672,7 +702,13 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
         rhs_v[s] = 0.0;
         w_v[s] = 0.0;
         zrow[s] = false;
-        if constexpr (MODE != SM_SPMV) rhs_v[s] = (double)a.rhs[gi[s]];
+        if constexpr (V == 1) {
+            // (a dependency tile may have no rows)
+            const size_t gm = (size_t)a.rowmap[r0 + max(min(kk[s], nrow - 1), 0)] * BS + r;
+            rhs_v[s] = (double)a.rhs[gm];
+        } else if constexpr (MODE != SM_SPMV) {
+            rhs_v[s] = (double)a.rhs[gi[s]];
+        }
         if constexpr ((RED & SR_W0) != 0) w_v[s] = (double)a.w0[gi[s]];
         if constexpr (MODE == SM_SUB_DINV || MODE == SM_SUB_Z) {
             const double *D = a.dinv + gi[s] * BS;
@@ -697,12 +733,27 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     // 56.0 us with it (measured).
     XVec<TX, BS> xg[XV];
     int xq[XV];
+    int xl[XV];
 #pragma unroll
     for (int j = 0; j < XV; ++j) {
         const int q = min(t + j * NT, nblk > 0 ? nblk - 1 : 0);
-        const int cb = nblk > 0 ? scol[q] : 0;
+        int cb = nblk > 0 ? scol[q] : 0;
+        xl[j] = -1;
+        if constexpr (V == 2) {
+            xl[j] = cb < 0 ? -1 - cb : -1;
+            cb = max(cb, 0);
+        }
         xq[j] = q;
         xg[j] = *reinterpret_cast<const XVec<TX, BS> *>((cb < a.split ? a.x_lo : a.x) + (size_t)cb * BS);
+    }
+    if constexpr (V == 2) {
+#pragma unroll
+        for (int j = 0; j < XV; ++j)
+#pragma unroll
+            for (int c = 0; c < BS; ++c) {
+                const TX lv = ldsx[max(xl[j], 0) * BS + c];
+                if (xl[j] >= 0) xg[j].v[c] = lv;
+            }
     }
 #pragma unroll
     for (int j = 0; j < XV; ++j)
@@ -779,10 +830,12 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     for (int s = 0; s < RS; ++s) {
         if (!has_row[s]) continue;
         const TX o = from_double<TX>(res[s]);
-        a.out[gi[s]] = o;
         if constexpr (KEEP) keep[t] = o;
-        if (a.out2) a.out2[gi[s]] = o;
-        if (a.out3) a.out3[gi[s]] = o;
+        if constexpr (V != 1) {
+            a.out[gi[s]] = o;
+            if (a.out2) a.out2[gi[s]] = o;
+            if (a.out3) a.out3[gi[s]] = o;
+        }
         const double od = (double)o;
         double v0 = 0.0, v1 = 0.0;
         if constexpr (RED == SR_W0) v0 = od * w_v[s];
@@ -1095,6 +1148,34 @@ __global__ __launch_bounds__(64 * BS) void k_tail3(StreamArgs<TV, TX> a, Tail3Ar
             b.out[(size_t)b.row[64 + k] * BS + r] = from_double<TX>(z);
         }
     }
+}
+
+// The first sweep's colours c and c + 1 in one launch (forward
+// preconditioner, unit diagonal).  The first a.ntiles workgroups run colour
+// c's tiles.  Each of the others runs one tile of colour c + 1 ("own" rows)
+// after recomputing, into LDS, y of the colour c rows its own rows read: a
+// dependency tile v (packed copies of those rows, built at analysis, with
+// values refreshed at each factorisation) whose results stay in LDS, then
+// the own tile g, whose column indices of those rows are LDS places.  Both
+// compute every row with k_stream's arithmetic, so the recomputed y equal
+// the ones colour c's tiles store, bit for bit.  MODEB: SM_SUB, or SM_SUB_Z
+// when colour c + 1 is colour C-2 and the extended tail follows.
+template <int BS, int MODEB, int ROWSA, typename TV, typename TX>
+__global__ __launch_bounds__(64 * BS) void k_lower2(StreamArgs<TV, TX> a, StreamArgs<TV, TX> v,
+                                                    StreamArgs<TV, TX> g) {
+    if (a.ra.st->done) return;
+    if ((int)blockIdx.x < a.ntiles) {
+        const int4 d = a.tile[a.tile0 + xcd_tile(blockIdx.x, a.ntiles)];
+        stream_tile<BS, SM_SUB, SR_NONE, TV, TX, ROWSA, false, plain_small<TV>()>(a, d, nullptr, 0);
+        return;
+    }
+    __shared__ TX ydep[64 * BS];
+    const int j = xcd_tile(blockIdx.x - a.ntiles, g.ntiles);
+    const int4 dv = v.tile[j];
+    const int4 dg = g.tile[j];
+    stream_tile<BS, SM_SUB, SR_NONE, TV, TX, 64, true, plain_small<TV>(), 1>(v, dv, ydep, 0);
+    __syncthreads();
+    stream_tile<BS, MODEB, SR_NONE, TV, TX, 64, false, plain_small<TV>(), 2>(g, dg, nullptr, 0, ydep);
 }
 
 // ---------------------------------------------------------------------------

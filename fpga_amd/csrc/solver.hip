@@ -88,6 +88,19 @@ struct DevMat {
     DBuf<int4> tail3_tile;
     int tail3_n = -1, tail3_ntiles = 0;
     bool tail3_tall = false;
+    // the first sweep's colours C-3 and C-2 in one launch (k_lower2), for a
+    // first-sweep pattern: low2_n tiles of colour C-2 rows (at most 64 rows,
+    // at most 64 colour C-3 rows read, each side's blocks within a tile's
+    // capacity), -1 when not built.  low2_tile: those tiles; low2_vtile:
+    // their dependency tiles, rows t*64 + k of a packed pattern (low2_vrowptr,
+    // low2_vcol, low2_vmap: the matrix row, low2_vsrc: the pattern's block
+    // index of each packed block, for the value copies); low2_col: colidx
+    // with the colour C-3 columns of colour C-2 rows replaced by -1 - their
+    // place in the tile's dependency tile.  Both lists in launch order.
+    DBuf<int4> low2_tile, low2_vtile;
+    DBuf<int32_t> low2_vrowptr, low2_vcol, low2_vmap, low2_vsrc, low2_col;
+    int low2_n = -1;
+    int64_t low2_nvblk = 0;
 
     // slices per layer in the launch order, one per XCD (measured on a
     // 128x128x64 grid: 4 slices no better, 16 and 32 slower, FP64 iteration
@@ -309,6 +322,119 @@ struct DevMat {
         tail3_n = n3;
     }
 
+    // Tiles of colour C-2 of a first-sweep pattern for k_lower2, each with the
+    // packed copy of the colour C-3 rows it reads.
+    void set_low2(const MatPattern &p, const std::vector<int32_t> &colour_ptr, const std::vector<int32_t> &o,
+                  int64_t span, int bs, hipStream_t s) {
+        low2_n = -1;
+        const int nc = (int)colour_ptr.size() - 1;
+        if (nc < 4) return;
+        const int c0 = colour_ptr[nc - 3], c1 = colour_ptr[nc - 2], c2 = colour_ptr[nc - 1];
+        const int cap = tile_cap(bs);
+        struct T {
+            int first, end;
+            std::vector<int32_t> deps;
+        };
+        std::vector<T> tiles;
+        std::vector<int32_t> mark(c1 - c0, -1);  // slot of a colour C-3 row in the tile being built
+        int r = c1;
+        while (r < c2) {
+            T t{r, r, {}};
+            int blocks = 0, dblocks = 0;
+            while (r < c2 && r - t.first < kTileRows) {
+                const int nb = p.rowptr[r + 1] - p.rowptr[r];
+                if (nb > cap) return;
+                int nnew = 0, bnew = 0;
+                for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q) {
+                    const int col = p.colidx[q];
+                    if (col >= c1) return;  // a first sweep only reads earlier colours
+                    if (col >= c0 && mark[col - c0] < 0) {
+                        mark[col - c0] = -2;  // counted for this row
+                        ++nnew;
+                        bnew += p.rowptr[col + 1] - p.rowptr[col];
+                    }
+                }
+                for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q) {
+                    const int col = p.colidx[q];
+                    if (col >= c0 && mark[col - c0] == -2) mark[col - c0] = -1;
+                }
+                const bool fits = blocks + nb <= cap && (int)t.deps.size() + nnew <= kTileRows && dblocks + bnew <= cap;
+                if (!fits) {
+                    if (r == t.first) return;  // one row alone does not fit
+                    break;
+                }
+                for (int32_t q = p.rowptr[r]; q < p.rowptr[r + 1]; ++q) {
+                    const int col = p.colidx[q];
+                    if (col >= c0 && mark[col - c0] < 0) {
+                        mark[col - c0] = (int)t.deps.size();
+                        t.deps.push_back(col);
+                    }
+                }
+                blocks += nb;
+                dblocks += bnew;
+                ++r;
+            }
+            t.end = r;
+            for (int32_t d : t.deps) mark[d - c0] = -1;
+            tiles.push_back(std::move(t));
+        }
+        // launch order
+        std::vector<int> ord(tiles.size());
+        for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+            return tile_key(o, p.n, span, tiles[x].first) < tile_key(o, p.n, span, tiles[y].first);
+        });
+        const int nt = (int)tiles.size();
+        std::vector<int4> td(std::max(nt, 1)), vd(std::max(nt, 1));
+        std::vector<int32_t> vrp((size_t)nt * kTileRows + 1, 0), vmap((size_t)nt * kTileRows, 0), vcol, vsrc;
+        std::vector<int32_t> col(p.colidx.begin(), p.colidx.end());
+        for (int k = 0; k < nt; ++k) {
+            const T &t = tiles[ord[k]];
+            td[k] = make_int4(t.first, p.rowptr[t.first], t.end, p.rowptr[t.end]);
+            const int vr0 = k * kTileRows;
+            for (int j = 0; j < kTileRows; ++j) {
+                if (j < (int)t.deps.size()) {
+                    const int d = t.deps[j];
+                    vmap[vr0 + j] = d;
+                    for (int32_t q = p.rowptr[d]; q < p.rowptr[d + 1]; ++q) {
+                        vcol.push_back(p.colidx[q]);
+                        vsrc.push_back(q);
+                    }
+                }
+                vrp[vr0 + j + 1] = (int32_t)vcol.size();
+            }
+            vd[k] = make_int4(vr0, vrp[vr0], vr0 + (int)t.deps.size(), vrp[vr0 + (int)t.deps.size()]);
+            for (int32_t q = p.rowptr[t.first]; q < p.rowptr[t.end]; ++q) {
+                const int c = p.colidx[q];
+                if (c >= c0 && c < c1) {
+                    const auto it = std::find(t.deps.begin(), t.deps.end(), c);
+                    col[q] = -1 - (int32_t)(it - t.deps.begin());
+                }
+            }
+        }
+        low2_nvblk = (int64_t)vcol.size();
+        for (int i = 0; i < kPadBlocks; ++i) vcol.push_back(0);
+        if (vsrc.empty()) vsrc.push_back(0);
+        for (int i = 0; i < kPadBlocks; ++i) col.push_back(0);
+        low2_tile.ensure(td.size());
+        low2_vtile.ensure(vd.size());
+        low2_vrowptr.ensure(vrp.size());
+        low2_vmap.ensure(std::max<size_t>(vmap.size(), 1));
+        low2_vcol.ensure(vcol.size());
+        low2_vsrc.ensure(vsrc.size());
+        low2_col.ensure(col.size());
+        HIP_CHECK(hipMemcpyAsync(low2_tile.p, td.data(), td.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(low2_vtile.p, vd.data(), vd.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(low2_vrowptr.p, vrp.data(), vrp.size() * 4, hipMemcpyHostToDevice, s));
+        if (!vmap.empty())
+            HIP_CHECK(hipMemcpyAsync(low2_vmap.p, vmap.data(), vmap.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(low2_vcol.p, vcol.data(), vcol.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(low2_vsrc.p, vsrc.data(), vsrc.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipMemcpyAsync(low2_col.p, col.data(), col.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_CHECK(hipStreamSynchronize(s));
+        low2_n = nt;
+    }
+
     void set_tail(const MatPattern &p, const std::vector<int32_t> &colour_ptr, hipStream_t s) {
         tail_ndep = -1;
         tail3_n = -1;
@@ -345,6 +471,10 @@ struct DevMat {
         for (DBuf<int32_t> *b : {&tail3_row, &tail3_blk, &tail3_src}) b->release();
         tail3_tile.release();
         tail3_n = -1;
+        for (DBuf<int32_t> *b : {&low2_vrowptr, &low2_vcol, &low2_vmap, &low2_vsrc, &low2_col}) b->release();
+        low2_tile.release();
+        low2_vtile.release();
+        low2_n = -1;
         src.release();
     }
 };
@@ -383,6 +513,9 @@ enum ProfKind { PK_SPMV = 0, PK_LOWER = 1, PK_UPPER = 2, PK_VEC = 3, PK_N = 4 };
 #ifndef BDA_TAIL3
 #define BDA_TAIL3 1
 #endif
+#ifndef BDA_LOW2
+#define BDA_LOW2 1
+#endif
 
 struct bda_handle_s {
     int bs = 0, n = 0;
@@ -405,11 +538,15 @@ struct bda_handle_s {
     bda::DBuf<int32_t> d_nonfinite;  // set by the factorisation when a matrix value is infinite or NaN
     bda::DBuf<double> d_orig, A64, L64, U64, LU, Dinv, DinvT, At64, Ut64, Lt64;
     bda::DBuf<float> A32, L32, U32;
+    bda::DBuf<double> low2_v64;  // values of L's packed dependency rows (k_lower2)
+    bda::DBuf<float> low2_v32;
     bda::DBuf<float> At32, Ut32, Lt32;  // FP32 transposes (component access only)
     uint64_t factor_gen = 0, mixed_gen = UINT64_MAX, adj_gen = UINT64_MAX, adj32_gen = UINT64_MAX;
     uint64_t values_gen = 0;  // changes with every upload, value update and factorisation
-    // the extended merged tail launch (k_tail3) where the colouring allows it
+    // the extended merged tail launch (k_tail3) and the merged lower launch
+    // (k_lower2) where the colouring allows them
     bool tail3 = BDA_TAIL3 != 0;
+    bool low2 = BDA_LOW2 != 0;
 
     // vectors
     bda::Krylov<double> k64;
@@ -819,6 +956,74 @@ void tail3_launch(H &h, const OpSet<TV> &op, const TX *y, const TX *lo, int spli
     else tail3_dispatch<BS, kTileRows, TV, TX>(h, a, b, c);
 }
 
+template <int BS, int MODEB, int ROWSA, typename TV, typename TX>
+void lower2_dispatch(H &h, const StreamArgs<TV, TX> &a, const StreamArgs<TV, TX> &v, const StreamArgs<TV, TX> &g) {
+    hipLaunchKernelGGL((k_lower2<BS, MODEB, ROWSA, TV, TX>), dim3(a.ntiles + g.ntiles), dim3(64 * BS), 0, h.stream, a,
+                       v, g);
+    check_launch();
+}
+
+// The first sweep's colours C-3 and C-2 in one launch (k_lower2) of a
+// forward application: y for both; with zout (the extended tail follows),
+// also z of colour C-2's rows without second-sweep blocks.
+template <int BS, typename TV, typename TX>
+void lower2_launch(H &h, const OpSet<TV> &op, const TV *vval, const TX *p, TX *y, const TX *lo, int split, TX *zout,
+                   bool use_state) {
+    const int nc = h.an.ncolours;
+    const DevMat &m = *op.S1;
+    const int c = nc - 3;
+    if (op.s1_dinv || m.low2_n < 0 || nc < 4) throw std::runtime_error("merged lower launch does not fit the colouring");
+    StreamArgs<TV, TX> a;
+    std::memset(&a, 0, sizeof(a));
+    a.rowptr = m.rowptr.p;
+    a.colidx = m.colidx.p;
+    a.val = op.S1val;
+    a.x = y;
+    a.x_lo = lo ? lo : y;
+    a.split = lo ? split : 0;
+    a.rhs = p;
+    a.out = y;
+    a.ra = red(h, 0, FIN_NONE);
+    if (!use_state) a.ra.st = h.st.p + kIdleState;
+    StreamArgs<TV, TX> v = a, g = a;
+    // colour c's own tiles, tall when that saves a quarter of the workgroups
+    a.tile0 = m.colour_tile[c];
+    a.ntiles = m.colour_tile[c + 1] - m.colour_tile[c];
+    a.tile = m.tile.p;
+    const int ntall = m.colour_tile_tall[c + 1] - m.colour_tile_tall[c];
+    const bool tall = 4 * ntall <= 3 * a.ntiles;
+    if (tall) {
+        a.tile = m.tile_tall.p;
+        a.tile0 = m.colour_tile_tall[c];
+        a.ntiles = ntall;
+    }
+    v.tile = m.low2_vtile.p;
+    v.tile0 = 0;
+    v.ntiles = m.low2_n;
+    v.rowptr = m.low2_vrowptr.p;
+    v.colidx = m.low2_vcol.p;
+    v.val = vval;
+    v.rowmap = m.low2_vmap.p;
+    g.tile = m.low2_tile.p;
+    g.tile0 = 0;
+    g.ntiles = m.low2_n;
+    g.colidx = m.low2_col.p;
+    if (zout) {
+        g.rowptr2 = op.S2->rowptr.p;
+        g.zout = zout;
+        g.dinv = op.dinv;
+        if (tall) lower2_dispatch<BS, SM_SUB_Z, kTileRowsTall, TV, TX>(h, a, v, g);
+        else lower2_dispatch<BS, SM_SUB_Z, kTileRows, TV, TX>(h, a, v, g);
+    } else {
+        if (tall) lower2_dispatch<BS, SM_SUB, kTileRowsTall, TV, TX>(h, a, v, g);
+        else lower2_dispatch<BS, SM_SUB, kTileRows, TV, TX>(h, a, v, g);
+    }
+}
+
+template <typename TV> const TV *low2_values(H &h);
+template <> const double *low2_values<double>(H &h) { return h.low2_v64.p; }
+template <> const float *low2_values<float>(H &h) { return h.low2_v32.p; }
+
 // z = M^{-1} p through the two colour sweeps of op (first sweep colours
 // ascending into y, second sweep colours descending into z).  Two launches
 // are saved per application:
@@ -842,9 +1047,13 @@ void precond(H &h, const OpSet<TV> &op, const TX *p, TX *y, TX *z, bool use_stat
     const bool merge = nc >= 3 && op.S2->tail_ndep >= 0 &&
                        op.S1->colour_tile[nc] - op.S1->colour_tile[nc - 1] == 1;
     const bool merge3 = merge && h.tail3 && !op.s1_dinv && op.s2_dinv && op.S2->tail3_n >= 0;
+    const bool low2 = h.low2 && skip0 && op.S1->low2_n >= 0 && nc >= 4 && op.S1 == &h.L;
     prof_begin(h, PK_LOWER);
     for (int c = skip0 ? 1 : 0; c < nc - 1; ++c) {
-        if (merge3 && c == nc - 2)
+        if (low2 && c == nc - 3) {
+            lower2_launch<BS, TV, TX>(h, op, low2_values<TV>(h), p, y, lo, split, merge3 ? z : nullptr, use_state);
+            ++c;  // colour C-2 is done too
+        } else if (merge3 && c == nc - 2)
             launch_stream<BS, SM_SUB_Z, SR_NONE, TV, TX>(h, *op.S1, op.S1val, c, y, p, y, op.dinv, nullptr,
                                                          red(h, 0, FIN_NONE), use_state, nullptr, nullptr, lo, split,
                                                          -1, op.S2->rowptr.p, z);
@@ -873,9 +1082,11 @@ int precond_launches(const H &h, const OpSet<TV> &op) {
     const bool merge = nc >= 3 && op.S2->tail_ndep >= 0 &&
                        op.S1->colour_tile[nc] - op.S1->colour_tile[nc - 1] == 1;
     const bool merge3 = merge && h.tail3 && !op.s1_dinv && op.s2_dinv && op.S2->tail3_n >= 0;
+    const bool low2 = h.low2 && skip0 && op.S1->low2_n >= 0 && nc >= 4 && op.S1 == &h.L;
     auto nonempty = [&](const DevMat &m, int c) { return m.colour_tile[c + 1] > m.colour_tile[c] ? 1 : 0; };
     int k = 0;
     for (int c = skip0 ? 1 : 0; c < nc - 1; ++c) k += nonempty(*op.S1, c);
+    if (low2) k -= 1;
     if (!merge) k += nonempty(*op.S1, nc - 1);
     if (merge || merge3) k += 1;
     for (int c = merge3 ? nc - 4 : merge ? nc - 3 : nc - 2; c >= 0; --c) k += nonempty(*op.S2, c);
@@ -1119,6 +1330,7 @@ void finish_analyse(H &h) {
     h.Lt.upload(an.Lt, h.stream, &an.perm, span);
     h.U.set_tail(an.U, an.colour_ptr, h.stream);
     h.U.set_tail3(an.U, an.colour_ptr, an.perm, span, h.bs, h.stream);
+    h.L.set_low2(an.L, an.colour_ptr, an.perm, span, h.bs, h.stream);
     h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
     h.d_diag.ensure(h.n);
@@ -1166,6 +1378,14 @@ void do_analyse(H &h, bool level) {
     finish_analyse(h);
 }
 
+// Values of L's packed dependency rows (k_lower2) from L64.
+template <int BS>
+void gather_low2(H &h) {
+    if (h.L.low2_n < 0) return;
+    h.low2_v64.ensure((h.L.low2_nvblk + kPadBlocks) * BS * BS);
+    gather_blocks<BS, double>(h, h.L.low2_nvblk, h.L.low2_vsrc.p, h.L64.p, h.low2_v64.p, 0);
+}
+
 template <int BS>
 void do_factor(H &h) {
     require_analysed(h);
@@ -1197,6 +1417,7 @@ void do_factor(H &h) {
     gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
     gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
     gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
+    gather_low2<BS>(h);
     HIP_CHECK(hipStreamSynchronize(h.stream));
     h.factored = true;
     h.factor_gen++;
@@ -1222,6 +1443,7 @@ void load_factors(H &h, const double *lu_values, const double *dinv) {
     gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
     gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
     gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
+    gather_low2<BS>(h);
     HIP_CHECK(hipStreamSynchronize(h.stream));
     h.factored = true;
     h.factor_gen++;
@@ -1239,6 +1461,10 @@ void ensure_mixed(H &h) {
     gather_blocks<BS, float>(h, h.A.nnzb, nullptr, h.A64.p, h.A32.p, 0);
     gather_blocks<BS, float>(h, h.L.nnzb, nullptr, h.L64.p, h.L32.p, 0);
     gather_blocks<BS, float>(h, h.U.nnzb, nullptr, h.U64.p, h.U32.p, 0);
+    if (h.L.low2_n >= 0) {
+        h.low2_v32.ensure((h.L.low2_nvblk + kPadBlocks) * bb);
+        gather_blocks<BS, float>(h, h.L.low2_nvblk, nullptr, h.low2_v64.p, h.low2_v32.p, 0);
+    }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     h.mixed_gen = h.factor_gen;
 }

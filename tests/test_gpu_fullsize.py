@@ -36,10 +36,11 @@ def big():
 
 def test_full_components_bit_exact(big):
     s, g, o = big
-    # five colours: lower sweeps of colours 1-3 (colour 3 also writing z where
-    # its upper sweep has no blocks), the extended tail (colour 4, colour 3's
-    # rows with blocks, colour 2), upper sweeps of colours 1 and 0
-    assert g.precond_launches() == 6
+    # five colours: the lower sweep of colour 1, those of colours 2 and 3 in
+    # one launch (colour 3 also writing z where its upper sweep has no
+    # blocks), the extended tail (colour 4, colour 3's rows with blocks,
+    # colour 2), the upper sweeps of colours 1 and 0
+    assert g.precond_launches() == 5
     for a, b in zip(g.colouring(), o.colouring()):
         np.testing.assert_array_equal(a, b)
     x = np.random.default_rng(7).uniform(-1, 1, s.N)

@@ -223,8 +223,10 @@ def test_precond_bit_exact_extended_tail(bs):
         g.factor()
         _, _, cptr = g.colouring()
         np.testing.assert_array_equal(np.diff(cptr), [1740, 200, 50, 10])
-        # lower sweeps of colours 1 and 2, the extended tail, the upper sweep of colour 0
-        assert g.precond_launches() == 4
+        # the lower sweeps of colours 1 and 2 in one launch (colour 2's tiles
+        # recompute the colour-1 rows they read), the extended tail, the upper
+        # sweep of colour 0
+        assert g.precond_launches() == 3
         o = oracle.OracleSystem(bs, rowptr, colidx, vals)
         o.factor()
         p = np.random.default_rng(9).uniform(-1, 1, (len(rowptr) - 1) * bs)
