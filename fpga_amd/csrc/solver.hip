@@ -322,6 +322,8 @@ struct DevMat {
         tail3_n = n3;
     }
 
+    static constexpr int kLow2MaxTiles = 2048;
+
     // Tiles of colour C-2 of a first-sweep pattern for k_lower2, each with the
     // packed copy of the colour C-3 rows it reads.
     void set_low2(const MatPattern &p, const std::vector<int32_t> &colour_ptr, const std::vector<int32_t> &o,
@@ -329,6 +331,11 @@ struct DevMat {
         low2_n = -1;
         const int nc = (int)colour_ptr.size() - 1;
         if (nc < 4) return;
+        // only where colour C-3's own launch is latency-bound (fewer tiles than
+        // the GPU holds at once): on the 4x4 256x256x64 grid (6,260 tiles) the
+        // recomputed rows cost more than the saved launch (FP64 26.1 + 10.4 us
+        // as two launches, 39.8 us merged)
+        if (p.colour_tile[nc - 2] - p.colour_tile[nc - 3] > kLow2MaxTiles) return;
         const int c0 = colour_ptr[nc - 3], c1 = colour_ptr[nc - 2], c2 = colour_ptr[nc - 1];
         const int cap = tile_cap(bs);
         struct T {

@@ -35,6 +35,10 @@ def big4():
 def test_full_bs4_components_bit_exact(big4):
     s, g, o = big4
     assert s.bs == 4
+    # one launch per colour and sweep: the last colour (84 rows) needs two
+    # tiles, so neither tail launch applies, and colour 2's launch (6,260
+    # tiles) streams at full rate, so colours 2 and 3 keep separate launches
+    assert g.precond_launches() == 8
     for a, b in zip(g.colouring(), o.colouring()):
         np.testing.assert_array_equal(a, b)
     x = np.random.default_rng(11).uniform(-1, 1, s.N)
