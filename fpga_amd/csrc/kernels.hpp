@@ -1375,9 +1375,14 @@ __global__ __launch_bounds__(kVecThreads) void k_init_zero_x0(int64_t N, const d
     block_partials<kVecThreads, 1>(vv, ra, blockIdx.x);
 }
 
-// x64 += d (outer refinement update of the precision form)
+// x64 += d (outer refinement update of the precision form), unless the inner
+// run produced no results: it then either never started (the outer loop had
+// already stopped when its start was queued) or had a zero right-hand side,
+// and d is not its output
 template <typename TXX>
-__global__ __launch_bounds__(kVecThreads) void k_add_correction(int64_t N, double *x, const TXX *d) {
+__global__ __launch_bounds__(kVecThreads) void k_add_correction(int64_t N, double *x, const TXX *d,
+                                                                const SolverState *in) {
+    if (in->no_results) return;
     const int64_t i = (int64_t)blockIdx.x * kVecThreads + threadIdx.x;
     if (i < N) x[i] = x[i] + (double)d[i];
 }
@@ -1420,6 +1425,17 @@ __global__ void k_gather_rows(int n, int bs, const int32_t *perm, const double *
     if (i >= (int64_t)n * bs) return;
     const int64_t k = i / bs, c = i - k * bs;
     out[i] = in[(int64_t)perm[k] * bs + c];
+}
+// two vectors in one launch (the right-hand side and the start vector)
+__global__ void k_gather_rows2(int n, int bs, const int32_t *perm, const double *in1, double *out1,
+                               const double *in2, double *out2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * bs) return;
+    const int64_t k = i / bs, c = i - k * bs;
+    const int64_t j = (int64_t)perm[k] * bs + c;
+    const double a = in1[j], b = in2[j];
+    out1[i] = a;
+    out2[i] = b;
 }
 __global__ void k_scatter_rows(int n, int bs, const int32_t *perm, const double *in, double *out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -1522,6 +1522,12 @@ OpSet<float> op_adj32(H &h) {
     return OpSet<float>{&h.At, h.At32.p, &h.Ut, h.Ut32.p, true, &h.Lt, h.Lt32.p, false, h.DinvT.p};
 }
 
+void gather_vec2(H &h, const double *in1, double *out1, const double *in2, double *out2) {
+    hipLaunchKernelGGL(k_gather_rows2, dim3(blocks_for((int64_t)h.N(), 256)), dim3(256), 0, h.stream, h.n, h.bs,
+                       h.d_perm.p, in1, out1, in2, out2);
+    check_launch();
+}
+
 void gather_vec(H &h, const double *in, double *out) {
     hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)h.N(), 256)), dim3(256), 0, h.stream, h.n, h.bs,
                        h.d_perm.p, in, out);
@@ -1544,8 +1550,7 @@ void solve_dev(H &h, const double *d_b, double *d_x, double tol, int maxit, bda_
     require_factored(h);
     check_solve_args(tol, maxit);
     if (adjoint) ensure_adjoint<BS>(h);
-    gather_vec(h, d_b, h.bp.p);
-    gather_vec(h, d_x, h.k64.x.p);
+    gather_vec2(h, d_b, h.bp.p, d_x, h.k64.x.p);
     OpSet<double> op = adjoint ? op_adj64(h) : op_fwd64(h);
     RunOut o = run_bicgstab<BS, double, double>(h, op, h.k64, h.bp.p, false, tol, maxit, h.abs_tol != 0);
     scatter_vec(h, h.k64.x.p, d_x);
@@ -1566,7 +1571,7 @@ bool mixed_inner(H &h, Krylov<TX> &k, int inner_maxit) {
     OpSet<float> op = op_fwd32(h);
     const RunOut o = run_bicgstab<BS, float, TX>(h, op, k, h.ro.p, true, 0.0, inner_maxit, false, h.st.p + 1);
     hipLaunchKernelGGL((k_add_correction<TX>), dim3(blocks_for((int64_t)h.N(), kVecThreads)), dim3(kVecThreads), 0,
-                       h.stream, (int64_t)h.N(), h.xo.p, (const TX *)k.x.p);
+                       h.stream, (int64_t)h.N(), h.xo.p, (const TX *)k.x.p, (const SolverState *)h.st.p);
     check_launch();
     hipLaunchKernelGGL(k_refine_step, dim3(1), dim3(1), 0, h.stream, h.st.p + 1, (const SolverState *)h.st.p);
     check_launch();
@@ -1583,8 +1588,7 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     ensure_mixed<BS>(h);
     if (vec32) h.k32.ensure(h.Npad());
     const auto t0 = std::chrono::steady_clock::now();
-    gather_vec(h, d_b, h.bp.p);
-    gather_vec(h, d_x, h.xo.p);
+    gather_vec2(h, d_b, h.bp.p, d_x, h.xo.p);
     SolverState os = new_state(h, 1, tol, 0, h.abs_tol != 0);
     os.max_outer = max_outer;
     os.inner_tol = inner_tol;
@@ -1607,11 +1611,18 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
                                                                nullptr, red(h, 1, FIN_REFINE), true);
         }
         if (outer == max_outer) break;
-        if (!wait_progress(h, 1, os.run_id, outer + 1, t0)) {
-            aborted = true;
-            break;
+        // After the first residual the host does not wait for the outer
+        // step's decision: the refinement stops there only when b is zero
+        // (or within the absolute tolerance), and then the inner run's start
+        // finds the outer state stopped and the whole inner run, the
+        // correction included, does nothing.
+        if (outer > 0) {
+            if (!wait_progress(h, 1, os.run_id, outer + 1, t0)) {
+                aborted = true;
+                break;
+            }
+            if (run_stopped(h, 1, os.run_id)) break;
         }
-        if (run_stopped(h, 1, os.run_id)) break;
         const bool ab = vec32 ? mixed_inner<BS, float>(h, h.k32, inner_maxit)
                               : mixed_inner<BS, double>(h, h.k64, inner_maxit);
         if (ab) {
