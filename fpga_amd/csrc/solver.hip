@@ -75,6 +75,7 @@ struct DevMat {
     // rows of colour C-2 that carry blocks (for the merged tail launch of a
     // second sweep), when there are at most kTailRows of them; else -1
     DBuf<int32_t> tail_dep;
+    std::vector<int32_t> tail_dep_rows;  // host copy
     int tail_ndep = -1;
     int tail_row0 = 0, tail_nrows = 0;
     // the extended merged tail (k_tail3), when also at most kTailRows rows of
@@ -261,8 +262,7 @@ struct DevMat {
         if (tail_ndep < 0 || nc < 3) return;
         const int c3_0 = colour_ptr[nc - 3], c3_1 = colour_ptr[nc - 2];
         const int last0 = colour_ptr[nc - 1];
-        std::vector<int32_t> dep(tail_ndep);
-        if (tail_ndep > 0) HIP_CHECK(hipMemcpy(dep.data(), tail_dep.p, tail_ndep * 4, hipMemcpyDeviceToHost));
+        const std::vector<int32_t> &dep = tail_dep_rows;
         std::vector<int32_t> dslot(tail_nrows, -1);
         for (int j = 0; j < tail_ndep; ++j) dslot[dep[j] - tail_row0] = j;
         constexpr int MB = kTail3Blocks;
@@ -464,6 +464,7 @@ struct DevMat {
             HIP_CHECK(hipMemcpyAsync(tail_dep.p, dep.data(), dep.size() * 4, hipMemcpyHostToDevice, s));
         HIP_CHECK(hipStreamSynchronize(s));
         tail_ndep = (int)dep.size();
+        tail_dep_rows = dep;
     }
     void release() {
         rowptr.release();
