@@ -123,6 +123,13 @@ struct DevMat {
         return w[w.size() / 2];
     }
 
+    // Launch-order key of a tile whose first row is r (below).
+    static int64_t tile_key(const std::vector<int32_t> &o, int64_t n, int64_t L, int r) {
+        const int64_t orow = o[r];
+        const int64_t slice = (L >= kLayerSlices && 2 * L <= n) ? ((orow % L) * kLayerSlices) / L : 0;
+        return slice * (n + 1) + orow;
+    }
+
     // Launch order of the tiles.  Every tile is independent and its
     // reduction partials go to a fixed slot, so the order has no effect on
     // results.  Whole-matrix launches take all tiles, per-colour launches
@@ -153,11 +160,7 @@ struct DevMat {
             const int64_t n = p.n;
             // key: slice of the layer (an eighth of L), then original row
             std::vector<int64_t> key(nt);
-            for (int t = 0; t < nt; ++t) {
-                const int64_t r = o[tile_row[t]];
-                const int64_t slice = (L >= kLayerSlices && 2 * L <= n) ? ((r % L) * kLayerSlices) / L : 0;
-                key[t] = slice * (n + 1) + r;
-            }
+            for (int t = 0; t < nt; ++t) key[t] = tile_key(o, n, L, tile_row[t]);
             auto by_key = [&](int a, int b) { return key[a] < key[b]; };
             std::vector<int32_t> ord(nt);
             // whole-matrix list
@@ -218,13 +221,6 @@ struct DevMat {
         upload_tiles(p, p.tile_row_tall, p.colour_tile_tall, orig_row, span, tile_tall, slot_tall, nullptr, nullptr,
                      s);
     }
-    // Launch-order key of a tile whose first row is r (upload_tiles).
-    static int64_t tile_key(const std::vector<int32_t> &o, int64_t n, int64_t L, int r) {
-        const int64_t orow = o[r];
-        const int64_t slice = (L >= kLayerSlices && 2 * L <= n) ? ((orow % L) * kLayerSlices) / L : 0;
-        return slice * (n + 1) + orow;
-    }
-
     // Tiles over the row ranges runs (as build_tiles does for a colour), in
     // launch order; false when a row has more blocks than a tile holds.
     static bool tiles_for_runs(const MatPattern &p, const std::vector<std::pair<int, int>> &runs, int cap,
