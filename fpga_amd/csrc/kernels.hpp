@@ -280,9 +280,40 @@ __global__ void k_refine_step(SolverState *out, const SolverState *in) {
         signal_done(out);
 }
 
+// Sum over the 64 lanes of a wave as the butterfly v += v[lane ^ o] for
+// o = 32, 16, 8, 4, 2, 1, with VALU lane moves instead of LDS-crossbar
+// shuffles: the permlane swaps exchange the wave's halves and the 16-lane
+// rows; DPP moves the rest.  row_ror:8 is lane ^ 8 exactly; at the o = 4
+// step every lane holds a sum that depends only on lane mod 8, and
+// row_ror:4 reads lane - 4 within the row, whose residue is (lane mod 8) ^ 4,
+// so each step adds the same two values as the shuffle butterfly and the
+// result is the same bit for bit (checked on the GPU against __shfl_xor over
+// 262,144 random sums).
+__device__ inline double lane_swap32(double v) {
+    const int lane = threadIdx.x & 63;
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return lane < 32 ? __hiloint2double(hi[1], lo[1]) : __hiloint2double(hi[0], lo[0]);
+}
+__device__ inline double lane_swap16(double v) {
+    const int lane = threadIdx.x & 63;
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+    return (lane & 16) == 0 ? __hiloint2double(hi[1], lo[1]) : __hiloint2double(hi[0], lo[0]);
+}
+template <int CTRL>
+__device__ inline double lane_dpp(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    v += lane_swap32(v);
+    v += lane_swap16(v);
+    v += lane_dpp<0x128>(v);  // row_ror:8
+    v += lane_dpp<0x124>(v);  // row_ror:4
+    v += lane_dpp<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+    v += lane_dpp<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
     return v;
 }
 
