@@ -117,12 +117,15 @@ void build_tiles(MatPattern &p, const std::vector<int32_t> &colour_ptr, int bs) 
     build_tiles(p, colour_ptr, tile_cap(bs, false), p.tile_row_tall, p.colour_tile_tall, kTileRowsTall);
 }
 
-void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {
+void validate_pattern(const Bcsr &m) {
     const int n = m.n;
     if (n <= 0) throw std::runtime_error("empty matrix");
     if ((int)m.rowptr.size() != n + 1 || m.rowptr[0] != 0) throw std::runtime_error("bad rowptr");
-    for (int i = 0; i < n; ++i) {
+    // the whole rowptr first, so that no column read below leaves colidx
+    for (int i = 0; i < n; ++i)
         if (m.rowptr[i + 1] < m.rowptr[i]) throw std::runtime_error("rowptr not monotone");
+    if ((size_t)m.rowptr[n] != m.colidx.size()) throw std::runtime_error("rowptr[n] is not the block count");
+    for (int i = 0; i < n; ++i) {
         for (int32_t q = m.rowptr[i]; q < m.rowptr[i + 1]; ++q) {
             const int32_t j = m.colidx[q];
             if (j < 0 || j >= n) throw std::runtime_error("column index out of range in row " + std::to_string(i));
@@ -130,6 +133,11 @@ void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {
                 throw std::runtime_error("column indices not strictly increasing in row " + std::to_string(i));
         }
     }
+}
+
+void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {
+    const int n = m.n;
+    validate_pattern(m);
     an.n = n;
     an.bs = m.bs;
     an.level_scheduling = level_scheduling;
@@ -139,22 +147,12 @@ void analyse(const Bcsr &m, bool level_scheduling, Analysis &an) {
 
 void analyse_given_order(const Bcsr &m, const std::vector<int32_t> &colour_ptr, Analysis &an) {
     const int n = m.n;
-    if (n <= 0) throw std::runtime_error("empty matrix");
-    if ((int)m.rowptr.size() != n + 1 || m.rowptr[0] != 0) throw std::runtime_error("bad rowptr");
+    validate_pattern(m);
     const int nc = (int)colour_ptr.size() - 1;
     if (nc < 1 || colour_ptr[0] != 0 || colour_ptr[nc] != n)
         throw std::runtime_error("colour_ptr must run from 0 to n");
     for (int c = 0; c < nc; ++c)
         if (colour_ptr[c + 1] <= colour_ptr[c]) throw std::runtime_error("every colour must hold at least one row");
-    for (int i = 0; i < n; ++i) {
-        if (m.rowptr[i + 1] < m.rowptr[i]) throw std::runtime_error("rowptr not monotone");
-        for (int32_t q = m.rowptr[i]; q < m.rowptr[i + 1]; ++q) {
-            const int32_t j = m.colidx[q];
-            if (j < 0 || j >= n) throw std::runtime_error("column index out of range in row " + std::to_string(i));
-            if (q > m.rowptr[i] && m.colidx[q - 1] >= j)
-                throw std::runtime_error("column indices not strictly increasing in row " + std::to_string(i));
-        }
-    }
     an.n = n;
     an.bs = m.bs;
     an.level_scheduling = false;

@@ -76,6 +76,10 @@ struct Analysis {
 // Returns the number of colours.
 int colour_rows(const Bcsr &m, bool level_scheduling, std::vector<int32_t> &colour);
 
+// Throws std::runtime_error unless m is a block CSR pattern of n > 0 rows:
+// rowptr of n + 1 nondecreasing entries from 0 to colidx.size(), and
+// strictly increasing column indices in [0, n) within each row.
+void validate_pattern(const Bcsr &m);
 // Full analysis; throws std::runtime_error on a bad pattern.
 void analyse(const Bcsr &m, bool level_scheduling, Analysis &an);
 // The same for a matrix whose rows are already in colour order, colour c

@@ -75,7 +75,11 @@ int bda_colour_pattern(int32_t n, const int32_t *rowptr, const int32_t *colidx, 
         m.bs = 3;
         m.n = n;
         m.rowptr.assign(rowptr, rowptr + n + 1);
+        if (rowptr[0] != 0 || rowptr[n] < 0) return 2;
+        for (int32_t i = 0; i < n; ++i)
+            if (rowptr[i + 1] < rowptr[i]) return 2;
         m.colidx.assign(colidx, colidx + rowptr[n]);
+        bda::validate_pattern(m);
         std::vector<int32_t> col;
         const int nc = bda::colour_rows(m, level_scheduling != 0, col);
         if (num_colours) *num_colours = nc;
