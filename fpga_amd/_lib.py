@@ -78,6 +78,8 @@ SIGNATURES = {
     "bda_colour_pattern": (i32, [i32, vp, vp, i32, vp, vp, vp, P_i32]),
     "bda_encode_status": (i32, [C.POINTER(BdaResult), vp]),
     "bda_encode_query": (i32, [C.POINTER(BdaLimits), i32, i64, i32, vp]),
+    "bda_set_debug_level": (i32, [i32]),
+    "bda_get_debug_level": (i32, []),
     "bda_create": (i32, [i32, i32, i32, C.POINTER(vp)]),
     "bda_destroy": (i32, [vp]),
     "bda_last_error": (C.c_char_p, [vp]),

@@ -112,4 +112,15 @@ struct Grid {
 
 void generate_grid(const GridSpec &spec, Grid &g);
 
+// Diagnostic lines on stderr ("INFO: ..." / "ERROR: ...", flushed at
+// once), printed when the process-wide debug level is at least `level`.
+// The level is read from the environment variable BDA_DEBUG_LEVEL at first
+// use and can be changed with bda_set_debug_level; 0 (the default) prints
+// nothing.  Level 1: handle creation, analysis and factor summaries, one
+// line per solve, errors; level 2 adds per-colour sizes, the launch plan
+// and every solve's norm ring.
+int debug_level();
+void set_debug_level(int level);
+void info(int level, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+
 }  // namespace bda

@@ -1,6 +1,10 @@
 // extern "C" access to the host-only parts: the synthetic grid generator and
 // the colouring / colour-ordered permutation (no device needed).
+#include <atomic>
 #include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <stdexcept>
@@ -13,7 +17,50 @@ struct bda_grid_s {
     bda::Grid g;
 };
 
+// Run-time counterpart of the reference's compile-time BDA_DEBUG_LEVEL
+// printouts (bda_utils.hpp:46-57): the same variable name, read from the
+// environment once, and a setter, so that a built library can be asked for
+// its diagnostics without a rebuild.
+namespace bda {
+namespace {
+std::atomic<int> g_debug_level{-1};
+}  // namespace
+
+int debug_level() {
+    int v = g_debug_level.load(std::memory_order_relaxed);
+    if (v < 0) {
+        const char *e = std::getenv("BDA_DEBUG_LEVEL");
+        v = e ? std::atoi(e) : 0;
+        v = v < 0 ? 0 : v;
+        int expected = -1;
+        if (!g_debug_level.compare_exchange_strong(expected, v)) v = expected;
+    }
+    return v;
+}
+
+void set_debug_level(int level) { g_debug_level.store(level < 0 ? 0 : level, std::memory_order_relaxed); }
+
+void info(int level, const char *fmt, ...) {
+    if (debug_level() < level) return;
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    std::fprintf(stderr, "%s\n", buf);
+    std::fflush(stderr);
+}
+}  // namespace bda
+
 extern "C" {
+
+int bda_set_debug_level(int32_t level) {
+    if (level < 0) return 2;
+    bda::set_debug_level(level);
+    return 0;
+}
+
+int bda_get_debug_level(void) { return bda::debug_level(); }
 
 int bda_grid_create(const bda_grid_spec *spec, bda_grid *out) {
     if (!spec || !out) return 2;

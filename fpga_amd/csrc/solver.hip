@@ -1323,6 +1323,8 @@ void permute_values(H &h) {
 
 // Device side of an analysis held in h.an: patterns, tilings, permutation
 // and the buffers sized by them.
+void log_analysis(H &h);
+
 void finish_analyse(H &h) {
     const Analysis &an = h.an;
     const int64_t span = DevMat::coupling_span(an.A, an.perm);
@@ -1374,6 +1376,7 @@ void finish_analyse(H &h) {
         if (h.bs == 3) permute_values<3>(h);
         else permute_values<4>(h);
     }
+    log_analysis(h);
 }
 
 void do_analyse(H &h, bool level) {
@@ -1397,6 +1400,7 @@ void do_factor(H &h) {
     constexpr int BB = BS * BS;
     const Analysis &an = h.an;
     const int32_t big = INT_MAX, zero = 0;
+    const auto t0 = std::chrono::steady_clock::now();
     HIP_CHECK(hipMemcpyAsync(h.d_bad.p, &big, 4, hipMemcpyHostToDevice, h.stream));
     HIP_CHECK(hipMemcpyAsync(h.d_nonfinite.p, &zero, 4, hipMemcpyHostToDevice, h.stream));
     {   // LU = A, recording whether A holds a non-finite value (for the step-0 residual's short path)
@@ -1418,6 +1422,8 @@ void do_factor(H &h) {
     if (bad != INT_MAX)
         throw std::runtime_error("singular diagonal block in ILU0 at colour-ordered row " + std::to_string(bad) +
                                  " (original row " + std::to_string(an.perm[bad]) + ")");
+    bda::info(1, "INFO: factor: block ILU0 of %d block rows over %d colours, %.3f ms", h.n, an.ncolours,
+              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     gather_blocks<BS, double>(h, h.L.nnzb, h.L.src.p, h.LU.p, h.L64.p, 0);
     gather_blocks<BS, double>(h, h.U.nnzb, h.U.src.p, h.LU.p, h.U64.p, 0);
     gather_blocks<BS, double>(h, h.n, nullptr, h.Dinv.p, h.DinvT.p, 1);
@@ -1519,6 +1525,39 @@ OpSet<float> op_adj32(H &h) {
     return OpSet<float>{&h.At, h.At32.p, &h.Ut, h.Ut32.p, true, &h.Lt, h.Lt32.p, false, h.DinvT.p};
 }
 
+// Diagnostics of an analysed system: sizes (level 1), then per colour and
+// the preconditioner's launch plan (level 2), the counterpart of the
+// reference's per-colour size printouts (fpga_functions_bicgstab.cpp:146-167).
+void log_analysis(H &h) {
+    if (bda::debug_level() < 1) return;
+    const Analysis &an = h.an;
+    bda::info(1, "INFO: analyse: %d block rows of %dx%d, %lld blocks, %d colours (%s); L %lld and U %lld blocks",
+              h.n, h.bs, h.bs, (long long)an.A.nnzb(), an.ncolours, an.level_scheduling ? "level sets" : "greedy",
+              (long long)an.L.nnzb(), (long long)an.U.nnzb());
+    if (bda::debug_level() < 2) return;
+    for (int c = 0; c < an.ncolours; ++c) {
+        const int r0 = an.colour_ptr[c], r1 = an.colour_ptr[c + 1];
+        bda::info(2, "INFO: analyse: colour %d: %d rows; A %d, L %d, U %d blocks; L %d and U %d tiles", c, r1 - r0,
+                  an.A.rowptr[r1] - an.A.rowptr[r0], an.L.rowptr[r1] - an.L.rowptr[r0],
+                  an.U.rowptr[r1] - an.U.rowptr[r0], h.L.colour_tile[c + 1] - h.L.colour_tile[c],
+                  h.U.colour_tile[c + 1] - h.U.colour_tile[c]);
+    }
+    bda::info(2, "INFO: analyse: %d SpMV tiles; preconditioner launches per application: %d, transposed %d",
+              h.A.ntiles, precond_launches(h, op_fwd64(h)), precond_launches(h, op_adj64(h)));
+}
+
+// One line per solve (level 1) and its norm ring (level 2).
+void log_result(const char *what, const bda_result &r) {
+    if (bda::debug_level() < 1) return;
+    bda::info(1,
+              "INFO: %s: %d half-iterations (%d refinement steps), rel residual %.3e; converged %d, no results %d, "
+              "breakdown %d, NaN %d, iteration cap %d, aborted %d; %.3f ms",
+              what, r.half_iterations, r.outer_iterations, r.rel_residual, r.converged, r.no_results, r.breakdown,
+              r.nan_detected, r.max_iter_reached, r.aborted, r.solve_ms);
+    bda::info(2, "INFO: %s: ||r0|| %.6e, last norms %.6e %.6e %.6e (newest %d)", what, r.norms[0], r.norms[1],
+              r.norms[2], r.norms[3], r.last_norm_idx);
+}
+
 void gather_vec2(H &h, const double *in1, double *out1, const double *in2, double *out2) {
     hipLaunchKernelGGL(k_gather_rows2, dim3(blocks_for((int64_t)h.N(), 256)), dim3(256), 0, h.stream, h.n, h.bs,
                        h.d_perm.p, in1, out1, in2, out2);
@@ -1552,7 +1591,10 @@ void solve_dev(H &h, const double *d_b, double *d_x, double tol, int maxit, bda_
     RunOut o = run_bicgstab<BS, double, double>(h, op, h.k64, h.bp.p, false, tol, maxit, h.abs_tol != 0);
     scatter_vec(h, h.k64.x.p, d_x);
     HIP_CHECK(hipStreamSynchronize(h.stream));
-    fill_result(o, res);
+    bda_result r;
+    fill_result(o, &r);
+    if (res) *res = r;
+    log_result(adjoint ? "adjoint solve" : "solve", r);
     h.last_solve_valid = true;
     h.last_kind = 0;
 }
@@ -1655,6 +1697,7 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     r.rel_residual = s.nrm0 > 0 ? s.last_norm / s.nrm0 : 0.0;
     r.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (res) *res = r;
+    log_result("precision-form solve", r);
     h.last_solve_valid = true;
     h.last_kind = 1;
 }
@@ -1922,9 +1965,11 @@ int guarded(bda_handle h, F &&f) {
         return 0;
     } catch (const std::exception &e) {
         h->err = e.what();
+        bda::info(1, "ERROR: %s", e.what());
         return 1;
     } catch (...) {
         h->err = "unknown error";
+        bda::info(1, "ERROR: unknown error");
         return 1;
     }
 }
@@ -1975,6 +2020,8 @@ int bda_create(int32_t bs, int32_t n, int32_t nnzb_max, bda_handle *out) {
         bda_destroy(h);
         return 1;
     }
+    bda::info(1, "INFO: bda_create: %dx%d blocks, %d block rows, room for %d blocks; device %d, %d compute units", bs,
+              bs, n, nnzb_max, h->device, h->cus);
     *out = h;
     return 0;
 }

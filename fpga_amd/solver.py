@@ -148,6 +148,20 @@ def colour_pattern(rowptr, colidx, level_scheduling: bool = False):
     return colour, perm, cptr[: nc.value + 1].copy()
 
 
+def set_debug_level(level: int) -> int:
+    """Set the library's diagnostic level (0 silent, 1 summaries and errors,
+    2 per-colour detail); returns the previous level.  The initial level is
+    the environment variable BDA_DEBUG_LEVEL."""
+    prev = int(lib.bda_get_debug_level())
+    if lib.bda_set_debug_level(int(level)) != 0:
+        raise BdaError(f"debug level must be >= 0, got {level}")
+    return prev
+
+
+def debug_level() -> int:
+    return int(lib.bda_get_debug_level())
+
+
 # --------------------------------------------------------------------------
 class BicgstabSolver:
     """One solver handle: one HIP stream, device buffers owned by the handle."""

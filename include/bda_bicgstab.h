@@ -120,6 +120,19 @@ int bda_encode_status(const bda_result *res, uint64_t line[16]);
  * handle of n block rows, nnzb_max blocks and block size bs. */
 int bda_encode_query(const bda_limits *limits, int32_t n, int64_t nnzb_max, int32_t bs, uint64_t line[8]);
 
+/* ---- diagnostics --------------------------------------------------------- */
+/* Process-wide level of the diagnostic lines the library prints on stderr
+ * ("INFO: ..." and "ERROR: ...", flushed at once): 0 none (the default),
+ * 1 handle creation, analysis and factor summaries, one line per solve and
+ * every error, 2 also per-colour sizes, the preconditioner's launch plan
+ * and the norm ring of each solve.  The initial level comes from the
+ * environment variable BDA_DEBUG_LEVEL.  Replaces the reference's
+ * compile-time BDA_DEBUG_LEVEL printouts (src/sda_app/common/bda_utils.hpp:
+ * 46-57; e.g. the per-colour sizes of fpga_functions_bicgstab.cpp:146-167).
+ * bda_set_debug_level returns 2 for a negative level. */
+int bda_set_debug_level(int32_t level);
+int bda_get_debug_level(void);
+
 /* ---- handle lifecycle -------------------------------------------------- */
 /* Allocate for a system of n block rows and at most nnzb_max blocks of
  * size bs x bs (bs in {3,4}).  Replaces fpga_setup_host_datamem +
