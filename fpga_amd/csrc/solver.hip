@@ -514,6 +514,11 @@ enum ProfKind { PK_SPMV = 0, PK_LOWER = 1, PK_UPPER = 2, PK_VEC = 3, PK_N = 4 };
 }  // namespace
 }  // namespace bda
 
+// Build switches of the merged sweep launches (k_tail3: the last colour's
+// first sweep with the second sweep of colours C-2 and C-3; k_lower2: the
+// first sweep of colours C-3 and C-2).  1, the default, merges where the
+// pattern allows it; 0 keeps the launches per colour (same results, for
+// A/B timing).
 #ifndef BDA_TAIL3
 #define BDA_TAIL3 1
 #endif
