@@ -642,6 +642,15 @@ static_assert(TileCap<3, false>::value == 168 && TileCap<4, false>::value == 128
 
 using H = bda_handle_s;
 
+// A copy on the handle's stream, complete on return.  It is ordered with the
+// handle's queued work like every other call; unlike hipMemcpy (the legacy
+// default stream) it does not wait for the work of other handles' streams,
+// so handles used from different threads do not serialise each other.
+void copy_sync(H &h, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, h.stream));
+    HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
 int blocks_for(int64_t work, int tpb) { return (int)((work + tpb - 1) / tpb); }
 
 void prof_begin(H &h, int kind) {
@@ -1276,7 +1285,7 @@ RunOut run_bicgstab(H &h, const OpSet<TV> &op, Krylov<TX> &k, const double *rhs,
     }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     out.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    HIP_CHECK(hipMemcpy(h.h_st, h.st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+    copy_sync(h, h.h_st, h.st.p, sizeof(SolverState), hipMemcpyDeviceToHost);
     out.st = *h.h_st;
     prof_collect(h, out.st.no_results ? -1 : out.st.h);
     h.prof_tag = -1;
@@ -1345,13 +1354,13 @@ void finish_analyse(H &h) {
     h.Lt.set_tail(an.Lt, an.colour_ptr, h.stream);
     h.d_perm.ensure(h.n);
     h.d_diag.ensure(h.n);
-    HIP_CHECK(hipMemcpy(h.d_perm.p, an.perm.data(), h.n * 4, hipMemcpyHostToDevice));
-    HIP_CHECK(hipMemcpy(h.d_diag.p, an.diag.data(), h.n * 4, hipMemcpyHostToDevice));
+    copy_sync(h, h.d_perm.p, an.perm.data(), h.n * 4, hipMemcpyHostToDevice);
+    copy_sync(h, h.d_diag.p, an.diag.data(), h.n * 4, hipMemcpyHostToDevice);
     h.blkrow.resize(an.A.nnzb());
     for (int k = 0; k < h.n; ++k)
         for (int32_t q = an.A.rowptr[k]; q < an.A.rowptr[k + 1]; ++q) h.blkrow[q] = k;
     h.d_blkrow.ensure(h.blkrow.size());
-    HIP_CHECK(hipMemcpy(h.d_blkrow.p, h.blkrow.data(), h.blkrow.size() * 4, hipMemcpyHostToDevice));
+    copy_sync(h, h.d_blkrow.p, h.blkrow.data(), h.blkrow.size() * 4, hipMemcpyHostToDevice);
 
     const size_t bb = (size_t)h.bs * h.bs;
     const int64_t nnzb = an.A.nnzb();
@@ -1682,7 +1691,7 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     scatter_vec(h, h.xo.p, d_x);
     HIP_CHECK(hipStreamSynchronize(h.stream));
     prof_collect(h);  // inner runs are not filtered: their no-op launches count too
-    HIP_CHECK(hipMemcpy(h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost));
+    copy_sync(h, h.h_st, h.st.p + 1, sizeof(SolverState), hipMemcpyDeviceToHost);
     const SolverState s = *h.h_st;
     bda_result r;
     std::memset(&r, 0, sizeof(r));
@@ -1778,7 +1787,7 @@ void spmv_comp(H &h, const TX *x, TX *y, int transpose, const TX *w, int redm, d
     HIP_CHECK(hipStreamSynchronize(h.stream));
     if (redm != 0 && dots) {
         SolverState s;
-        HIP_CHECK(hipMemcpy(&s, st, sizeof(SolverState), hipMemcpyDeviceToHost));
+        copy_sync(h, &s, st, sizeof(SolverState), hipMemcpyDeviceToHost);
         dots[0] = s.tot[0];
         dots[1] = redm == 3 ? s.tot[1] : 0.0;
     }
@@ -1819,10 +1828,10 @@ template <int BS>
 void spmv_perm(H &h, int prec, const void *x, void *y, int transpose, const void *w, int redm, double *dots) {
     comp_prepare<BS>(h, prec, transpose);
     const size_t es = prec == PREC_F32 ? 4 : 8;
-    HIP_CHECK(hipMemcpy(h.vin.p, x, h.N() * es, hipMemcpyHostToDevice));
-    if (w) HIP_CHECK(hipMemcpy(h.vw.p, w, h.N() * es, hipMemcpyHostToDevice));
+    copy_sync(h, h.vin.p, x, h.N() * es, hipMemcpyHostToDevice);
+    if (w) copy_sync(h, h.vw.p, w, h.N() * es, hipMemcpyHostToDevice);
     spmv_comp_prec<BS>(h, prec, h.vin.p, h.vout.p, transpose, w ? h.vw.p : nullptr, redm, dots);
-    HIP_CHECK(hipMemcpy(y, h.vout.p, h.N() * es, hipMemcpyDeviceToHost));
+    copy_sync(h, y, h.vout.p, h.N() * es, hipMemcpyDeviceToHost);
 }
 
 // The same on the caller's device vectors.
@@ -1870,7 +1879,7 @@ void precond_perm(H &h, int prec, const void *p, void *z, int transpose) {
     require_factored(h);
     comp_prepare<BS>(h, prec, transpose);
     const size_t es = prec == PREC_F32 ? 4 : 8;
-    HIP_CHECK(hipMemcpy(h.vin.p, p, h.N() * es, hipMemcpyHostToDevice));
+    copy_sync(h, h.vin.p, p, h.N() * es, hipMemcpyHostToDevice);
     // the intermediate of the two sweeps goes to vw
     if (prec == PREC_F32) {
         OpSet<float> op = transpose ? op_adj32(h) : op_fwd32(h);
@@ -1884,7 +1893,7 @@ void precond_perm(H &h, int prec, const void *p, void *z, int transpose) {
     }
     HIP_CHECK(hipStreamSynchronize(h.stream));
     prof_collect(h);
-    HIP_CHECK(hipMemcpy(z, h.vout.p, h.N() * es, hipMemcpyDeviceToHost));
+    copy_sync(h, z, h.vout.p, h.N() * es, hipMemcpyDeviceToHost);
 }
 
 template <int BS>
@@ -2096,7 +2105,7 @@ int bda_upload_bcsr(bda_handle h, const int32_t *rowptr, const int32_t *colidx, 
         h->have_pattern = true;
         h->analysed = false;
         h->factored = false;
-        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice);
         h->have_values = true;
         h->values_gen++;
     });
@@ -2106,7 +2115,7 @@ int bda_update_values(bda_handle h, const double *values) {
     return guarded(h, [&] {
         if (!h->have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
         const int64_t nnzb = h->pattern.nnzb();
-        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice);
         h->have_values = true;
         h->factored = false;
         h->values_gen++;
@@ -2118,7 +2127,7 @@ int bda_update_values_dev(bda_handle h, const double *d_values) {
     return guarded(h, [&] {
         if (!h->have_pattern) throw std::runtime_error("bda_upload_bcsr has not been called");
         const int64_t nnzb = h->pattern.nnzb();
-        HIP_CHECK(hipMemcpy(h->d_orig.p, d_values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyDeviceToDevice));
+        copy_sync(*h, h->d_orig.p, d_values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyDeviceToDevice);
         h->have_values = true;
         h->factored = false;
         h->values_gen++;
@@ -2148,7 +2157,7 @@ int bda_upload_factored(bda_handle h, int32_t ncolours, const int32_t *colour_pt
         h->pattern = m;
         h->have_pattern = true;
         h->an = std::move(an);
-        HIP_CHECK(hipMemcpy(h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->d_orig.p, values, (size_t)nnzb * h->bs * h->bs * 8, hipMemcpyHostToDevice);
         h->have_values = true;
         h->values_gen++;
         finish_analyse(*h);   // identity permutation: A's values are copied as given
@@ -2215,10 +2224,10 @@ int bda_solve(bda_handle h, const double *b, double *x, double tol, int32_t maxi
     return guarded(h, [&] {
         if (!b || !x) throw std::runtime_error("null vector");
         require_factored(*h);
-        HIP_CHECK(hipMemcpy(h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice);
+        copy_sync(*h, h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice);
         BS_CALL(h, solve_dev, *h, h->vin.p, h->vout.p, tol, maxit, res, false);
-        HIP_CHECK(hipMemcpy(x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+        copy_sync(*h, x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2229,13 +2238,13 @@ int bda_get_residual(bda_handle h, double *r) {
         if (h->last_kind == 1) {
             src = h->ro.p;
         } else {
-            HIP_CHECK(hipMemcpy(h->h_st, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+            copy_sync(*h, h->h_st, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost);
             const int lh = h->h_st->last_half;
             src = lh == 1 ? h->k64.s.p : lh == 2 ? h->k64.r.p : h->k64.rt.p;
         }
         scatter_vec(*h, src, h->vout.p);
         HIP_CHECK(hipStreamSynchronize(h->stream));
-        HIP_CHECK(hipMemcpy(r, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+        copy_sync(*h, r, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2251,10 +2260,10 @@ int bda_solve_mixed(bda_handle h, const double *b, double *x, double tol, int32_
     return guarded(h, [&] {
         if (!b || !x) throw std::runtime_error("null vector");
         require_factored(*h);
-        HIP_CHECK(hipMemcpy(h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->vin.p, b, h->N() * 8, hipMemcpyHostToDevice);
+        copy_sync(*h, h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice);
         BS_CALL(h, solve_mixed_dev, *h, h->vin.p, h->vout.p, tol, max_outer, inner_tol, inner_maxit, vec32, res);
-        HIP_CHECK(hipMemcpy(x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+        copy_sync(*h, x, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2267,10 +2276,10 @@ int bda_adjoint_solve(bda_handle h, const double *w, double *lambda, double tol,
     return guarded(h, [&] {
         if (!w || !lambda) throw std::runtime_error("null vector");
         require_factored(*h);
-        HIP_CHECK(hipMemcpy(h->vin.p, w, h->N() * 8, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(h->vout.p, lambda, h->N() * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->vin.p, w, h->N() * 8, hipMemcpyHostToDevice);
+        copy_sync(*h, h->vout.p, lambda, h->N() * 8, hipMemcpyHostToDevice);
         BS_CALL(h, solve_dev, *h, h->vin.p, h->vout.p, tol, maxit, res, true);
-        HIP_CHECK(hipMemcpy(lambda, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost));
+        copy_sync(*h, lambda, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2284,10 +2293,10 @@ int bda_grad_values(bda_handle h, const double *lambda, const double *x, double 
         require_analysed(*h);
         const size_t nv = (size_t)h->A.nnzb * h->bs * h->bs;
         h->ggrad.ensure(nv);
-        HIP_CHECK(hipMemcpy(h->vin.p, lambda, h->N() * 8, hipMemcpyHostToDevice));
-        HIP_CHECK(hipMemcpy(h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice));
+        copy_sync(*h, h->vin.p, lambda, h->N() * 8, hipMemcpyHostToDevice);
+        copy_sync(*h, h->vout.p, x, h->N() * 8, hipMemcpyHostToDevice);
         BS_CALL(h, grad_dev, *h, h->vin.p, h->vout.p, h->ggrad.p);
-        HIP_CHECK(hipMemcpy(grad, h->ggrad.p, nv * 8, hipMemcpyDeviceToHost));
+        copy_sync(*h, grad, h->ggrad.p, nv * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2363,10 +2372,10 @@ int bda_get_factors(bda_handle h, int32_t *l_rowptr, int32_t *l_colidx, double *
         if (u_rowptr) std::memcpy(u_rowptr, an.U.rowptr.data(), (h->n + 1) * 4);
         if (u_colidx) std::memcpy(u_colidx, an.U.colidx.data(), an.U.colidx.size() * 4);
         if (l_values && an.L.nnzb() > 0)
-            HIP_CHECK(hipMemcpy(l_values, h->L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost));
+            copy_sync(*h, l_values, h->L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost);
         if (u_values && an.U.nnzb() > 0)
-            HIP_CHECK(hipMemcpy(u_values, h->U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost));
-        if (dinv) HIP_CHECK(hipMemcpy(dinv, h->Dinv.p, (size_t)h->n * bb * 8, hipMemcpyDeviceToHost));
+            copy_sync(*h, u_values, h->U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost);
+        if (dinv) copy_sync(*h, dinv, h->Dinv.p, (size_t)h->n * bb * 8, hipMemcpyDeviceToHost);
     });
 }
 
@@ -2384,10 +2393,10 @@ void wire_inputs(H &h, WireInputs &w) {
     w.l.resize(std::max<size_t>(an.L.nnzb() * bb, 1));
     w.u.resize(std::max<size_t>(an.U.nnzb() * bb, 1));
     w.dinv.resize((size_t)h.n * bb);
-    HIP_CHECK(hipMemcpy(w.a.data(), h.A64.p, w.a.size() * 8, hipMemcpyDeviceToHost));
-    if (an.L.nnzb()) HIP_CHECK(hipMemcpy(w.l.data(), h.L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost));
-    if (an.U.nnzb()) HIP_CHECK(hipMemcpy(w.u.data(), h.U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(w.dinv.data(), h.Dinv.p, w.dinv.size() * 8, hipMemcpyDeviceToHost));
+    copy_sync(h, w.a.data(), h.A64.p, w.a.size() * 8, hipMemcpyDeviceToHost);
+    if (an.L.nnzb()) copy_sync(h, w.l.data(), h.L64.p, an.L.nnzb() * bb * 8, hipMemcpyDeviceToHost);
+    if (an.U.nnzb()) copy_sync(h, w.u.data(), h.U64.p, an.U.nnzb() * bb * 8, hipMemcpyDeviceToHost);
+    copy_sync(h, w.dinv.data(), h.Dinv.p, w.dinv.size() * 8, hipMemcpyDeviceToHost);
 }
 std::vector<double> to_colour_order(const H &h, const double *v) {
     std::vector<double> out((size_t)h.n * h.bs, 0.0);
@@ -2440,10 +2449,10 @@ int bda_get_history(bda_handle h, double *norms, int32_t capacity, int32_t *coun
         if (!h->last_solve_valid) throw std::runtime_error("no solve has run");
         HIP_CHECK(hipStreamSynchronize(h->stream));
         SolverState s;
-        HIP_CHECK(hipMemcpy(&s, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost));
+        copy_sync(*h, &s, h->st.p, sizeof(SolverState), hipMemcpyDeviceToHost);
         const int32_t avail = std::min(s.nhist, h->hist_cap);
         const int32_t n = std::max(0, std::min(avail, capacity));
-        if (n > 0 && norms) HIP_CHECK(hipMemcpy(norms, h->hist.p, (size_t)n * 8, hipMemcpyDeviceToHost));
+        if (n > 0 && norms) copy_sync(*h, norms, h->hist.p, (size_t)n * 8, hipMemcpyDeviceToHost);
         if (count) *count = s.nhist;
     });
 }

@@ -80,3 +80,51 @@ def test_host_lead_does_not_change_results(system, poll_iters):
         np.testing.assert_array_equal(x3, x4)
     finally:
         g.close()
+
+
+def test_handles_on_two_threads():
+    """One handle per thread, as the reference's callers use it: two threads
+    solving two different systems at the same time (each on its own handle
+    and stream; the ctypes calls release the GIL) get the same iteration
+    counts and bit-identical x as the same solves run alone."""
+    import threading
+    from fpga_amd.configs import GRID_128x128x64, scaled
+    specs = [fa.configs.GRID_32x32x16, scaled(GRID_128x128x64, 20, 18, 10)]
+    systems = [fa.generate(sp) for sp in specs]
+    handles = [make(s) for s in systems]
+    try:
+        solo = []
+        for s, g in zip(systems, handles):
+            x, r = g.solve(s.b, tol=1e-8, maxit=500)
+            xm, rm = g.solve_mixed(s.b, tol=1e-8)
+            solo.append((x, r.half_iterations, xm, rm.inner_half_iterations))
+        out = [[] for _ in handles]
+        errors = []
+
+        def work(k):
+            try:
+                s, g = systems[k], handles[k]
+                for _ in range(3):
+                    x, r = g.solve(s.b, tol=1e-8, maxit=500)
+                    xm, rm = g.solve_mixed(s.b, tol=1e-8)
+                    out[k].append((x, r.half_iterations, xm, rm.inner_half_iterations))
+            except Exception as e:  # reported below, on the main thread
+                errors.append(e)
+
+        print("case two threads, two handles", flush=True)
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join(timeout=300)
+        assert not any(t.is_alive() for t in ts)
+        assert not errors, errors
+        for k in range(2):
+            assert len(out[k]) == 3
+            for x, h, xm, hm in out[k]:
+                assert h == solo[k][1] and hm == solo[k][3]
+                np.testing.assert_array_equal(x, solo[k][0])
+                np.testing.assert_array_equal(xm, solo[k][2])
+    finally:
+        for g in handles:
+            g.close()
