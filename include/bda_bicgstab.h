@@ -24,7 +24,9 @@
  *     the stream is a blocking stream, so "_dev" inputs written by work on
  *     the legacy default stream are complete before the handle reads them;
  *     inputs written on other streams must be synchronised by the caller;
- *   - a handle must not be used from two threads at once.
+ *   - a handle must not be used from two threads at once; handles used from
+ *     different threads run concurrently (the library's copies are on the
+ *     handle's stream, never on the legacy default stream).
  *
  * Vectors passed to bda_solve*, bda_adjoint_solve* and bda_grad_values are in
  * the caller's ORIGINAL row ordering; the handle permutes internally.
