@@ -492,13 +492,14 @@ __global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const doub
 //
 // One workgroup (64*BS threads, BS waves) handles one tile: up to 64 (or,
 // in the tall tiling, 128) consecutive block rows of one colour whose blocks
-// are contiguous in the value array.  Phase 1 streams the tile's values with
-// one lane per matrix entry (7 blocks of 3x3 = 63 lanes, or 4 blocks of 4x4
-// = 64 lanes per wave step), so every wave load instruction reads
-// consecutive memory; the tile's x entries are gathered into LDS (column
-// indices staged there first), each lane multiplies its entry, and the BS
-// lanes of one block row combine their products left to right with
-// cross-lane shuffles.  The per-block row partials go to LDS.  Phase 2 gives
+// are contiguous in the value array.  Phase 1 streams the tile's values,
+// with FP32 values one lane per block row (BS values, one vector load), with
+// FP64 values one lane per matrix entry (7 blocks of 3x3 = 63 lanes, or 4
+// blocks of 4x4 = 64 lanes per wave step); either way every wave load
+// instruction reads consecutive memory.  The tile's x entries are gathered
+// into LDS (column indices staged there first); a row lane adds its block
+// row's products left to right, entry lanes combine them in the same order
+// with cross-lane shifts.  The per-block row partials go to LDS.  Phase 2 gives
 // one thread per scalar row (two in the tall tiling), which adds its block
 // partials in column order and applies the epilogue.
 //
@@ -633,6 +634,23 @@ template <int RED> struct NRedOf { static constexpr int value = (RED & 1) + ((RE
 // ROWS = 128 (the tall tiling, used by sweep colours with few blocks per row)
 // gives every thread two rows of the epilogue, 64 rows apart.
 //
+// Value layout of the streaming kernel's lanes.  Row lanes: one lane per
+// block row, its BS values one vector load, the row's products added in the
+// lane.  Entry lanes: one lane per matrix entry, the products of a block row
+// added across its BS lanes with DPP shifts.  Both add the same products in
+// the same order, so the results are the same bit for bit.  Measured on the
+// 128x128x64 3x3 grid (A/B, two rounds): with FP32 values row lanes take
+// the SpMV from 53.7 to 50.1 us and the preconditioner application from 70.3
+// to 65.7 us (49 instead of 65 VGPRs); with FP64 values they are slower (SpMV
+// 93 -> 110.5 us), so FP64 values keep entry lanes.  BDA_ROW_LANES = 0 or 1
+// forces one layout for every value type (A/B builds).
+#ifndef BDA_ROW_LANES
+#define BDA_ROW_LANES -1
+#endif
+template <typename TV> constexpr bool row_lanes() {
+    return BDA_ROW_LANES < 0 ? std::is_same<TV, float>::value : BDA_ROW_LANES != 0;
+}
+
 template <typename TX, int BS, int CAP>
 struct TileLds {
     double part[CAP * BS];
@@ -705,15 +723,33 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     for (int j = 0; j < CJ; ++j) ci[j] = load_mat<PLAIN>(&a.colidx[b0 + min(t + j * NT, nblk)]);
     const int rp = a.rowptr[r0 + min(t, nrow)];
 
-    // 2. values: slot u of this lane is block (u*NW + wave)*BPW + lb, entry e
+    // 2. values.  Row lanes: slot u of thread t is block row j = u*NT + t of
+    // the tile (block j / BS, row j % BS), its BS values one vector load;
+    // the tile's block rows are contiguous, so a wave reads 64 consecutive
+    // rows.  Entry lanes: slot u of this lane is block (u*NW + wave)*BPW + lb,
+    // entry e.
+    constexpr bool RL = row_lanes<TV>();
+    constexpr int SR = (CAP * BS + NT - 1) / NT;  // row lanes: block rows per thread
+    constexpr int NS = RL ? SR : S;               // value slots per thread
+    constexpr int NC = RL ? BS : 1;               // values per slot
     const int lb = lane / BB, e = lane - lb * BB, er = e / BS, ec = e - er * BS;
     const bool active_lane = lane < VPW;
-    const TV *val = a.val + (size_t)b0 * BB + e;
-    TV av[S];
+    TV av[NS][NC];
+    if constexpr (RL) {
+        const TV *valr = a.val + (size_t)b0 * BB;
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-        const int bl = (u * NW + wave) * BPW + lb;
-        av[u] = load_mat<PLAIN>(&val[(size_t)min(bl, nblk) * BB]);
+        for (int u = 0; u < NS; ++u) {
+            const int jc = min(u * NT + t, nblk * BS);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) av[u][c] = load_mat<PLAIN>(&valr[(size_t)jc * BS + c]);
+        }
+    } else {
+        const TV *val = a.val + (size_t)b0 * BB + e;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int bl = (u * NW + wave) * BPW + lb;
+            av[u][0] = load_mat<PLAIN>(&val[(size_t)min(bl, nblk) * BB]);
+        }
     }
 
     // 3. per-row operands of the epilogue: thread t serves scalar row r of
@@ -793,24 +829,43 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     __syncthreads();
 
     // all LDS reads first (clamped, unconditional), then the products
-    TX xs[S];
+    TX xs[NS][NC];
+    if constexpr (RL) {
 #pragma unroll
-    for (int u = 0; u < S; ++u) xs[u] = sx[min((u * NW + wave) * BPW + lb, CAP - 1) * BS + ec];
+        for (int u = 0; u < NS; ++u) {
+            const int bl = min((u * NT + t) / BS, CAP - 1);
 #pragma unroll
-    for (int u = 0; u < S; ++u) {
-        const int bl = (u * NW + wave) * BPW + lb;
-        const bool ok = active_lane && bl < nblk;
-        const double p = prod<TV, TX>(ok ? av[u] : (TV)0, ok ? xs[u] : (TX)0);
-        double sm = p;
-        const double p1 = lane_down1(p);
-        const double p2 = lane_down1(p1);
-        sm = sm + p1;
-        sm = sm + p2;
-        if constexpr (BS == 4) {
-            const double p3 = lane_down1(p2);
-            sm = sm + p3;
+            for (int c = 0; c < NC; ++c) xs[u][c] = sx[bl * BS + c];
         }
-        if (ok && ec == 0) part[bl * BS + er] = sm;
+        // a block row's products added left to right, as the entry lanes do
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int j = u * NT + t;
+            const bool ok = j / BS < nblk;
+            double sm = prod<TV, TX>(ok ? av[u][0] : (TV)0, ok ? xs[u][0] : (TX)0);
+#pragma unroll
+            for (int c = 1; c < NC; ++c) sm = sm + prod<TV, TX>(ok ? av[u][c] : (TV)0, ok ? xs[u][c] : (TX)0);
+            if (ok) part[j] = sm;
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < NS; ++u) xs[u][0] = sx[min((u * NW + wave) * BPW + lb, CAP - 1) * BS + ec];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int bl = (u * NW + wave) * BPW + lb;
+            const bool ok = active_lane && bl < nblk;
+            const double p = prod<TV, TX>(ok ? av[u][0] : (TV)0, ok ? xs[u][0] : (TX)0);
+            double sm = p;
+            const double p1 = lane_down1(p);
+            const double p2 = lane_down1(p1);
+            sm = sm + p1;
+            sm = sm + p2;
+            if constexpr (BS == 4) {
+                const double p3 = lane_down1(p2);
+                sm = sm + p3;
+            }
+            if (ok && ec == 0) part[bl * BS + er] = sm;
+        }
     }
     __syncthreads();
 
