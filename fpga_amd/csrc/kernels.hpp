@@ -494,12 +494,13 @@ __global__ __launch_bounds__(256) void k_copy_check_finite(int64_t n, const doub
 // in the tall tiling, 128) consecutive block rows of one colour whose blocks
 // are contiguous in the value array.  Phase 1 streams the tile's values,
 // with FP32 values one lane per block row (BS values, one vector load), with
-// FP64 values one lane per matrix entry (7 blocks of 3x3 = 63 lanes, or 4
-// blocks of 4x4 = 64 lanes per wave step); either way every wave load
+// FP64 values one lane per two consecutive entries (one 16-byte load; 63
+// lanes per wave step for 3x3, 64 for 4x4); either way every wave load
 // instruction reads consecutive memory.  The tile's x entries are gathered
 // into LDS (column indices staged there first); a row lane adds its block
-// row's products left to right, entry lanes combine them in the same order
-// with cross-lane shifts.  The per-block row partials go to LDS.  Phase 2 gives
+// row's products left to right, pair lanes add theirs with the next lane's
+// in the same order through cross-lane shifts.  The per-block row partials
+// go to LDS.  Phase 2 gives
 // one thread per scalar row (two in the tall tiling), which adds its block
 // partials in column order and applies the epilogue.
 //
@@ -636,19 +637,28 @@ template <int RED> struct NRedOf { static constexpr int value = (RED & 1) + ((RE
 //
 // Value layout of the streaming kernel's lanes.  Row lanes: one lane per
 // block row, its BS values one vector load, the row's products added in the
-// lane.  Entry lanes: one lane per matrix entry, the products of a block row
-// added across its BS lanes with DPP shifts.  Both add the same products in
-// the same order, so the results are the same bit for bit.  Measured on the
-// 128x128x64 3x3 grid (A/B, two rounds): with FP32 values row lanes take
-// the SpMV from 53.7 to 50.1 us and the preconditioner application from 70.3
-// to 65.7 us (49 instead of 65 VGPRs); with FP64 values they are slower (SpMV
-// 93 -> 110.5 us), so FP64 values keep entry lanes.  BDA_ROW_LANES = 0 or 1
-// forces one layout for every value type (A/B builds).
-#ifndef BDA_ROW_LANES
-#define BDA_ROW_LANES -1
+// lane.  Pair lanes: one lane per two consecutive matrix entries (one
+// 16-byte load, a wave's loads contiguous), a block row's products added
+// with the next lane's through DPP shifts.  Entry lanes: one lane per matrix
+// entry, the products of a block row added across its BS lanes with DPP
+// shifts.  All add the same products in the same order, so the results are
+// the same bit for bit.  Measured on the 128x128x64 3x3 grid (A/B, two
+// rounds): with FP32 values row lanes take the SpMV from 53.7 to 50.1 us and
+// the preconditioner application from 70.3 to 65.7 us (49 instead of 65
+// VGPRs).  With FP64 values row lanes are slower than entry lanes (SpMV 93 ->
+// 110.5 us: a lane's 24 bytes take two or three strided loads) and pair
+// lanes faster (SpMV 93.5 -> 91.0 us, application 110 -> 104.5 us, 91
+// instead of 102 VGPRs; the FP64 solve 16.87 -> 16.40 ms).  BDA_LANES32 /
+// BDA_LANES64 select the layout for FP32 / FP64 values.
+enum LaneLayout : int { LAY_ENTRY = 0, LAY_ROW = 1, LAY_PAIR = 2 };
+#ifndef BDA_LANES32
+#define BDA_LANES32 LAY_ROW
 #endif
-template <typename TV> constexpr bool row_lanes() {
-    return BDA_ROW_LANES < 0 ? std::is_same<TV, float>::value : BDA_ROW_LANES != 0;
+#ifndef BDA_LANES64
+#define BDA_LANES64 LAY_PAIR
+#endif
+template <typename TV> constexpr int lane_layout() {
+    return std::is_same<TV, float>::value ? BDA_LANES32 : BDA_LANES64;
 }
 
 template <typename TX, int BS, int CAP>
@@ -726,22 +736,34 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
     // 2. values.  Row lanes: slot u of thread t is block row j = u*NT + t of
     // the tile (block j / BS, row j % BS), its BS values one vector load;
     // the tile's block rows are contiguous, so a wave reads 64 consecutive
-    // rows.  Entry lanes: slot u of this lane is block (u*NW + wave)*BPW + lb,
-    // entry e.
-    constexpr bool RL = row_lanes<TV>();
+    // rows.  Pair lanes: slot u of lane l < PPW is entry pair q = (u*NW +
+    // wave)*PPW + l, entries 2q and 2q + 1 of the tile.  Entry lanes: slot u
+    // of this lane is block (u*NW + wave)*BPW + lb, entry e.
+    constexpr int LAY = lane_layout<TV>();
     constexpr int SR = (CAP * BS + NT - 1) / NT;  // row lanes: block rows per thread
-    constexpr int NS = RL ? SR : S;               // value slots per thread
-    constexpr int NC = RL ? BS : 1;               // values per slot
+    constexpr int PPW = BS == 3 ? 63 : 64;        // pair lanes: pairs per wave step
+    constexpr int SP = CAP * BB / 2 / (NW * PPW); // pair lanes: pairs per lane
+    static_assert(LAY != LAY_PAIR || (CAP * BB) % (2 * NW * PPW) == 0, "whole pair slots");
+    constexpr int NS = LAY == LAY_ROW ? SR : LAY == LAY_PAIR ? SP : S;  // value slots per thread
+    constexpr int NC = LAY == LAY_ROW ? BS : LAY == LAY_PAIR ? 2 : 1;   // values per slot
     const int lb = lane / BB, e = lane - lb * BB, er = e / BS, ec = e - er * BS;
     const bool active_lane = lane < VPW;
     TV av[NS][NC];
-    if constexpr (RL) {
+    if constexpr (LAY == LAY_ROW) {
         const TV *valr = a.val + (size_t)b0 * BB;
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
             const int jc = min(u * NT + t, nblk * BS);
 #pragma unroll
             for (int c = 0; c < NC; ++c) av[u][c] = load_mat<PLAIN>(&valr[(size_t)jc * BS + c]);
+        }
+    } else if constexpr (LAY == LAY_PAIR) {
+        const TV *valr = a.val + (size_t)b0 * BB;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int qc = min((u * NW + wave) * PPW + lane, nblk * BB / 2);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) av[u][c] = load_mat<PLAIN>(&valr[(size_t)qc * 2 + c]);
         }
     } else {
         const TV *val = a.val + (size_t)b0 * BB + e;
@@ -830,7 +852,7 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
 
     // all LDS reads first (clamped, unconditional), then the products
     TX xs[NS][NC];
-    if constexpr (RL) {
+    if constexpr (LAY == LAY_ROW) {
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
             const int bl = min((u * NT + t) / BS, CAP - 1);
@@ -846,6 +868,40 @@ __device__ __attribute__((always_inline)) inline void stream_tile(const StreamAr
 #pragma unroll
             for (int c = 1; c < NC; ++c) sm = sm + prod<TV, TX>(ok ? av[u][c] : (TV)0, ok ? xs[u][c] : (TX)0);
             if (ok) part[j] = sm;
+        }
+    } else if constexpr (LAY == LAY_PAIR) {
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int q = (u * NW + wave) * PPW + lane;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const int en = 2 * q + c;
+                xs[u][c] = sx[min(en / BB, CAP - 1) * BS + en % BS];
+            }
+        }
+        // 3x3: lanes 3k, 3k+1, 3k+2 hold entries 6k'..6k'+5, two block rows;
+        // lane 3k adds row 2k' (its two products, then the next lane's
+        // first), lane 3k+1 row 2k'+1 (its second product, then the next
+        // lane's two).  4x4: lane 2k adds its two products, then the next
+        // lane's two.  Each is ((p0 + p1) + p2) [+ p3], the entry lanes' order.
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int q = (u * NW + wave) * PPW + lane;
+            const double pa = prod<TV, TX>(av[u][0], xs[u][0]);
+            const double pb = prod<TV, TX>(av[u][1], xs[u][1]);
+            const double na = lane_down1(pa);
+            const double nb = lane_down1(pb);
+            int row;
+            double sm;
+            if constexpr (BS == 3) {
+                const int m = lane % 3;
+                row = m == 0 ? 2 * q / 3 : m == 1 ? (2 * q + 1) / 3 : -1;
+                sm = m == 0 ? (pa + pb) + na : (pb + na) + nb;
+            } else {
+                row = (lane & 1) == 0 ? q / 2 : -1;
+                sm = ((pa + pb) + na) + nb;
+            }
+            if (lane < PPW && row >= 0 && row < nblk * BS) part[row] = sm;
         }
     } else {
 #pragma unroll
