@@ -17,9 +17,10 @@ Beside the headline the line carries, measured in the same run:
   trisolve   back-to-back preconditioner applications (all L and U sweep
              launches), FP64 and FP32, against their algorithmic bytes;
   adjoint    the 64x64x32 grid's adjoint backward (A^T lambda = w to 1e-10,
-             then the gradient -lambda x^T on the pattern);
+             then the gradient -lambda x^T on the pattern), FP64 and in the
+             precision form (with lambda's difference from the FP64 one);
   grid256x256x64  the 4x4-block grid: FP64 forward, precision-form forward and
-             adjoint backward, each with its time and algorithmic GB/s;
+             adjoint backward (FP64 and precision form), each with its time;
   cpu_baseline    the CPU oracle on the host cores: the 128x128x64 solve (all
              threads, median of 3; one thread, one full solve) and the
              32x32x16 solve of the CPU-runnable case (median of 5, all threads
@@ -308,12 +309,28 @@ def adjoint_block(fa, torch, dev, steps):
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     by = iteration_bytes(g, fa, s, False) * ra.iterations + grad_bytes(s)
+    lam_ref = lam.clone()
+    nref = torch.linalg.vector_norm(lam_ref).item()
+    g.adjoint_solve_mixed_dev(w.data_ptr(), lam.data_ptr(), 1e-10)   # builds the FP32 transposes
+    times_m, rm = [], None
+    for _ in range(steps):
+        lam.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rm = g.adjoint_solve_mixed_dev(w.data_ptr(), lam.data_ptr(), 1e-10)
+        g.grad_values_dev(lam.data_ptr(), x.data_ptr(), grad.data_ptr())
+        torch.cuda.synchronize()
+        times_m.append(time.perf_counter() - t0)
+    tm = float(np.median(times_m))
+    em = torch.linalg.vector_norm(lam - lam_ref).item() / nref
     out = {"workload": "grid64x64x32", "block_rows": s.n, "blocks": s.nnzb, "time_s": t, "tol": 1e-10,
            "iterations": ra.iterations, "converged": ra.converged, "rel_residual": ra.rel_residual,
            "GBs_algorithmic": by / t / 1e9, "steps": steps,
+           "mixed": {"time_s": tm, "outer": rm.outer_iterations, "inner_iterations": rm.iterations,
+                     "converged": rm.converged, "lambda_rel_diff_vs_fp64": em, "speedup_vs_fp64": t / tm},
            "note": "175 MB resident set: cache-resident, GB/s above HBM peak is possible"}
     g.close()
-    log(f"adjoint grid64x64x32: {t * 1e3:.2f} ms, {ra.iterations} iterations")
+    log(f"adjoint grid64x64x32: {t * 1e3:.2f} ms, {ra.iterations} iterations; precision form {tm * 1e3:.2f} ms")
     return out
 
 
@@ -359,6 +376,17 @@ def big_block(fa, torch, dev, steps):
         return r
 
     tad, rad = timed(adj)
+    lam_ref = lam.clone()
+    nlam = torch.linalg.vector_norm(lam_ref).item()
+
+    def adj_mixed():
+        lam.zero_()
+        r = g.adjoint_solve_mixed_dev(w.data_ptr(), lam.data_ptr(), 1e-10)
+        g.grad_values_dev(lam.data_ptr(), x_ref.data_ptr(), grad.data_ptr())
+        return r
+
+    tam, ram = timed(adj_mixed)
+    eam = torch.linalg.vector_norm(lam - lam_ref).item() / nlam
     it64, it32 = iteration_bytes(g, fa, s, False), iteration_bytes(g, fa, s, True)
     out = {"workload": "grid256x256x64", "block": s.bs, "block_rows": s.n, "blocks": s.nnzb, "unknowns": s.N,
            "colours": g.num_colours, "setup_s": st, "steps": steps,
@@ -368,9 +396,13 @@ def big_block(fa, torch, dev, steps):
                      "x_rel_err_vs_1e-10": emx, "speedup_vs_fp64": t64 / tmx,
                      "GBs_algorithmic_approx": it32 * rmx.iterations / tmx / 1e9},
            "adjoint": {"time_s": tad, "tol": 1e-10, "iterations": rad.iterations, "converged": rad.converged,
-                       "GBs_algorithmic": (it64 * rad.iterations + grad_bytes(s)) / tad / 1e9}}
+                       "GBs_algorithmic": (it64 * rad.iterations + grad_bytes(s)) / tad / 1e9},
+           "adjoint_mixed": {"time_s": tam, "tol": 1e-10, "outer": ram.outer_iterations,
+                             "inner_iterations": ram.iterations, "converged": ram.converged,
+                             "lambda_rel_diff_vs_fp64": eam, "speedup_vs_fp64_adjoint": tad / tam}}
     g.close()
-    log(f"grid256x256x64: fp64 {t64 * 1e3:.1f} ms, mixed {tmx * 1e3:.1f} ms, adjoint {tad * 1e3:.1f} ms")
+    log(f"grid256x256x64: fp64 {t64 * 1e3:.1f} ms, mixed {tmx * 1e3:.1f} ms, adjoint {tad * 1e3:.1f} ms, "
+        f"precision-form adjoint {tam * 1e3:.1f} ms")
     return out
 
 

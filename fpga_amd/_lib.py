@@ -103,6 +103,8 @@ SIGNATURES = {
     "bda_set_refinement": (i32, [vp, i32, f64]),
     "bda_adjoint_solve": (i32, [vp, vp, vp, f64, i32, C.POINTER(BdaResult)]),
     "bda_adjoint_solve_dev": (i32, [vp, vp, vp, f64, i32, C.POINTER(BdaResult)]),
+    "bda_adjoint_solve_mixed": (i32, [vp, vp, vp, f64, i32, f64, i32, i32, C.POINTER(BdaResult)]),
+    "bda_adjoint_solve_mixed_dev": (i32, [vp, vp, vp, f64, i32, f64, i32, i32, C.POINTER(BdaResult)]),
     "bda_grad_values": (i32, [vp, vp, vp, vp]),
     "bda_grad_values_dev": (i32, [vp, vp, vp, vp]),
     "bda_spmv_perm": (i32, [vp, vp, vp, i32]),

@@ -1620,8 +1620,8 @@ void solve_dev(H &h, const double *d_b, double *d_x, double tol, int maxit, bda_
 // per outer step, after the residual, to learn whether another inner run is
 // needed.
 template <int BS, typename TX>
-bool mixed_inner(H &h, Krylov<TX> &k, int inner_maxit) {
-    OpSet<float> op = op_fwd32(h);
+bool mixed_inner(H &h, Krylov<TX> &k, int inner_maxit, bool adjoint) {
+    OpSet<float> op = adjoint ? op_adj32(h) : op_fwd32(h);
     const RunOut o = run_bicgstab<BS, float, TX>(h, op, k, h.ro.p, true, 0.0, inner_maxit, false, h.st.p + 1);
     hipLaunchKernelGGL((k_add_correction<TX>), dim3(blocks_for((int64_t)h.N(), kVecThreads)), dim3(kVecThreads), 0,
                        h.stream, (int64_t)h.N(), h.xo.p, (const TX *)k.x.p, (const SolverState *)h.st.p);
@@ -1631,14 +1631,19 @@ bool mixed_inner(H &h, Krylov<TX> &k, int inner_maxit) {
     return o.aborted;
 }
 
+// adjoint: the same refinement for A^T lambda = w, with the transposed
+// operators (FP64 A^T for the residual, FP32 A^T, U^T, L^T for the inner runs).
 template <int BS>
 void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_outer, double inner_tol,
-                     int inner_maxit, int vec32, bda_result *res) {
+                     int inner_maxit, int vec32, bda_result *res, bool adjoint = false) {
     require_factored(h);
     check_solve_args(tol, inner_maxit);
     check_solve_args(inner_tol, 0);
     if (max_outer < 0) throw std::runtime_error("max_outer must be >= 0");
-    ensure_mixed<BS>(h);
+    if (adjoint) ensure_adjoint32<BS>(h);
+    else ensure_mixed<BS>(h);
+    const DevMat &M = adjoint ? h.At : h.A;
+    const double *Mval = adjoint ? h.At64.p : h.A64.p;
     if (vec32) h.k32.ensure(h.Npad());
     const auto t0 = std::chrono::steady_clock::now();
     gather_vec2(h, d_b, h.bp.p, d_x, h.xo.p);
@@ -1657,10 +1662,10 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
             hipLaunchKernelGGL((k_mark_nonzero<double>), dim3(vec_blocks(h.N())), dim3(kVecThreads), 0, h.stream,
                                (int64_t)h.N(), (const double *)h.xo.p, (const int32_t *)h.d_nonfinite.p, h.st.p + 1);
             check_launch();
-            launch_stream<BS, SM_SUB, SR_SELF, double, double, true>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p,
+            launch_stream<BS, SM_SUB, SR_SELF, double, double, true>(h, M, Mval, -1, h.xo.p, h.bp.p, h.ro.p,
                                                                      nullptr, nullptr, red(h, 1, FIN_REFINE), true);
         } else {
-            launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, h.A, h.A64.p, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
+            launch_stream<BS, SM_SUB, SR_SELF, double, double>(h, M, Mval, -1, h.xo.p, h.bp.p, h.ro.p, nullptr,
                                                                nullptr, red(h, 1, FIN_REFINE), true);
         }
         if (outer == max_outer) break;
@@ -1676,8 +1681,8 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
             }
             if (run_stopped(h, 1, os.run_id)) break;
         }
-        const bool ab = vec32 ? mixed_inner<BS, float>(h, h.k32, inner_maxit)
-                              : mixed_inner<BS, double>(h, h.k64, inner_maxit);
+        const bool ab = vec32 ? mixed_inner<BS, float>(h, h.k32, inner_maxit, adjoint)
+                              : mixed_inner<BS, double>(h, h.k64, inner_maxit, adjoint);
         if (ab) {
             aborted = true;
             break;
@@ -1711,7 +1716,7 @@ void solve_mixed_dev(H &h, const double *d_b, double *d_x, double tol, int max_o
     r.rel_residual = s.nrm0 > 0 ? s.last_norm / s.nrm0 : 0.0;
     r.solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (res) *res = r;
-    log_result("precision-form solve", r);
+    log_result(adjoint ? "precision-form adjoint solve" : "precision-form solve", r);
     h.last_solve_valid = true;
     h.last_kind = 1;
 }
@@ -2279,6 +2284,26 @@ int bda_adjoint_solve(bda_handle h, const double *w, double *lambda, double tol,
         copy_sync(*h, h->vin.p, w, h->N() * 8, hipMemcpyHostToDevice);
         copy_sync(*h, h->vout.p, lambda, h->N() * 8, hipMemcpyHostToDevice);
         BS_CALL(h, solve_dev, *h, h->vin.p, h->vout.p, tol, maxit, res, true);
+        copy_sync(*h, lambda, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
+    });
+}
+
+int bda_adjoint_solve_mixed_dev(bda_handle h, const double *d_w, double *d_lambda, double tol, int32_t max_outer,
+                                double inner_tol, int32_t inner_maxit, int32_t vec32, bda_result *res) {
+    return guarded(h, [&] {
+        BS_CALL(h, solve_mixed_dev, *h, d_w, d_lambda, tol, max_outer, inner_tol, inner_maxit, vec32, res, true);
+    });
+}
+
+int bda_adjoint_solve_mixed(bda_handle h, const double *w, double *lambda, double tol, int32_t max_outer,
+                            double inner_tol, int32_t inner_maxit, int32_t vec32, bda_result *res) {
+    return guarded(h, [&] {
+        if (!w || !lambda) throw std::runtime_error("null vector");
+        require_factored(*h);
+        copy_sync(*h, h->vin.p, w, h->N() * 8, hipMemcpyHostToDevice);
+        copy_sync(*h, h->vout.p, lambda, h->N() * 8, hipMemcpyHostToDevice);
+        BS_CALL(h, solve_mixed_dev, *h, h->vin.p, h->vout.p, tol, max_outer, inner_tol, inner_maxit, vec32, res,
+                true);
         copy_sync(*h, lambda, h->vout.p, h->N() * 8, hipMemcpyDeviceToHost);
     });
 }

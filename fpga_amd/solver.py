@@ -320,6 +320,24 @@ class BicgstabSolver:
                                               C.byref(r)), "bda_adjoint_solve_dev")
         return SolveResult.from_c(r)
 
+    def adjoint_solve_mixed(self, w, lam0=None, tol: float = 1e-10, max_outer: int = 20, inner_tol: float = 1e-4,
+                            inner_maxit: int = 1000, vec32: bool = True):
+        """A^T lambda = w in the precision form (FP64 refinement around FP32-value inner solves)."""
+        w = _f64(w)
+        lam = np.zeros(self.N) if lam0 is None else _f64(lam0).copy()
+        r = BdaResult()
+        self._check(lib.bda_adjoint_solve_mixed(self._h, _p(w), _p(lam), tol, max_outer, inner_tol, inner_maxit,
+                                                int(vec32), C.byref(r)), "bda_adjoint_solve_mixed")
+        return lam, SolveResult.from_c(r)
+
+    def adjoint_solve_mixed_dev(self, w_ptr: int, lam_ptr: int, tol: float = 1e-10, max_outer: int = 20,
+                                inner_tol: float = 1e-4, inner_maxit: int = 1000, vec32: bool = True) -> SolveResult:
+        r = BdaResult()
+        self._check(lib.bda_adjoint_solve_mixed_dev(self._h, C.c_void_p(w_ptr), C.c_void_p(lam_ptr), tol, max_outer,
+                                                    inner_tol, inner_maxit, int(vec32), C.byref(r)),
+                    "bda_adjoint_solve_mixed_dev")
+        return SolveResult.from_c(r)
+
     def grad_values(self, lam, x) -> np.ndarray:
         lam, x = _f64(lam), _f64(x)
         nnzb = int(self._rowptr[-1])

@@ -248,6 +248,16 @@ int bda_adjoint_solve(bda_handle h, const double *w, double *lambda, double tol,
                       int32_t maxit, bda_result *res);
 int bda_adjoint_solve_dev(bda_handle h, const double *d_w, double *d_lambda, double tol,
                           int32_t maxit, bda_result *res);
+/* The adjoint in the precision form: the refinement of bda_solve_mixed
+ * (FP64 residual w - A^T lambda, FP32-value inner BiCGSTAB on the transposed
+ * ILU0) for A^T lambda = w; the same arguments and refinement policy.  No
+ * reference counterpart. */
+int bda_adjoint_solve_mixed(bda_handle h, const double *w, double *lambda, double tol,
+                            int32_t max_outer, double inner_tol, int32_t inner_maxit,
+                            int32_t vec32, bda_result *res);
+int bda_adjoint_solve_mixed_dev(bda_handle h, const double *d_w, double *d_lambda, double tol,
+                                int32_t max_outer, double inner_tol, int32_t inner_maxit,
+                                int32_t vec32, bda_result *res);
 /* grad[nnzb*bs*bs] (original block layout) = -lambda_i x_j^T on every block
  * (i,j) of the pattern: dL/dA for L(x(A)) with dL/dx = w. */
 int bda_grad_values(bda_handle h, const double *lambda, const double *x, double *grad);

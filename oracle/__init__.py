@@ -117,6 +117,7 @@ def lib():
             "orc_solve": [_vp, _vp, _vp, _f64, _i32, _i32, C.POINTER(OrcResult)],
             "orc_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, _i32, _f64, C.POINTER(OrcResult)],
             "orc_adjoint_solve": [_vp, _vp, _vp, _f64, _i32, C.POINTER(OrcResult)],
+            "orc_adjoint_solve_mixed": [_vp, _vp, _vp, _f64, _i32, _f64, _i32, _i32, _i32, _f64, C.POINTER(OrcResult)],
             "orc_grad": [_vp, _vp, _vp, _vp],
             "orc_get_history": [_vp, _vp, _i32, C.POINTER(_i32)],
         }.items():
@@ -266,6 +267,16 @@ class OracleSystem:
         r = OrcResult()
         if lib().orc_adjoint_solve(self._h, _p(w), _p(lam), tol, maxit, C.byref(r)) != 0:
             raise RuntimeError("orc_adjoint_solve failed")
+        return lam, r.as_dict()
+
+    def adjoint_solve_mixed(self, w, lam0=None, tol=1e-10, max_outer=20, inner_tol=1e-4, inner_maxit=1000,
+                            vec32=True, adaptive=True, eta=0.9):
+        w = np.ascontiguousarray(w, np.float64)
+        lam = np.zeros(self.N) if lam0 is None else np.array(lam0, np.float64)
+        r = OrcResult()
+        if lib().orc_adjoint_solve_mixed(self._h, _p(w), _p(lam), tol, max_outer, inner_tol, inner_maxit, int(vec32),
+                                         int(adaptive), eta, C.byref(r)) != 0:
+            raise RuntimeError("orc_adjoint_solve_mixed failed")
         return lam, r.as_dict()
 
     def grad(self, lam, x):

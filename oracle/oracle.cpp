@@ -744,21 +744,26 @@ int orc_adjoint_solve(orc_handle h, const double *w, double *lam, double tol, in
 
 template <typename TX>
 static RunState mixed_inner(orc_handle h, const double *rhs, std::vector<double> &xo, double inner_tol,
-                            int inner_maxit) {
+                            int inner_maxit, bool adjoint) {
     const int64_t N = (int64_t)h->n * h->bs;
-    OpSet<float> op{&h->A32, &h->L32, &h->U32, nullptr, h->dinv.data(), &h->cptr};
+    const OpSet<float> op = adjoint ? OpSet<float>{&h->At32, &h->Ut32, &h->Lt32, h->dinvT.data(), nullptr, &h->cptr}
+                                    : OpSet<float>{&h->A32, &h->L32, &h->U32, nullptr, h->dinv.data(), &h->cptr};
     std::vector<TX> d(N);
     RunState st = bicgstab<float, TX>(op, N, rhs, d.data(), true, inner_tol, inner_maxit, false);
     for (int64_t i = 0; i < N; ++i) xo[i] = xo[i] + (double)d[i];
     return st;
 }
 
-extern "C" {
-
-int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
-                    int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res) {
+// The precision form's refinement for A x = b, or (adjoint) A^T x = b with
+// the transposed operators: FP64 A^T for the residual, FP32 transposes for
+// the inner runs.
+static int solve_mixed_common(orc_handle h, const double *b, double *x, double tol, int32_t max_outer,
+                              double inner_tol, int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta,
+                              bool adjoint, orc_result *res) {
     if (!h->factored) return 1;
-    ensure_mixed(h);
+    if (adjoint) ensure_adj32(h);
+    else ensure_mixed(h);
+    const BMat<double> &M = adjoint ? h->At : h->A;
     const int64_t N = (int64_t)h->n * h->bs;
     std::vector<double> bp(N), xo(N), ro(N), v(N);
     permute_in(h, b, bp.data());
@@ -769,7 +774,7 @@ int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_
     double nrm0 = 0, thr = 0;
     int slot = 1;
     for (int outer = 0;; ++outer) {
-        spmv(h->A, xo.data(), v.data());
+        spmv(M, xo.data(), v.data());
         for (int64_t i = 0; i < N; ++i) ro[i] = bp[i] - v[i];
         const double nrm = std::sqrt(dot(N, ro.data(), ro.data()));
         if (outer == 0) {
@@ -792,8 +797,8 @@ int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_
         if (outer == max_outer) { r.max_iter_reached = 1; break; }
         // adaptive: ask only the reduction the outer target still needs
         const double itol = adaptive ? std::min(0.5, std::max(inner_tol, eta * thr / nrm)) : inner_tol;
-        RunState st = vec32 ? mixed_inner<float>(h, ro.data(), xo, itol, inner_maxit)
-                            : mixed_inner<double>(h, ro.data(), xo, itol, inner_maxit);
+        RunState st = vec32 ? mixed_inner<float>(h, ro.data(), xo, itol, inner_maxit, adjoint)
+                            : mixed_inner<double>(h, ro.data(), xo, itol, inner_maxit, adjoint);
         r.outer_iterations++;
         r.inner_half_iterations += st.h;
         r.iterations += st.h / 2.0 + 0.5;
@@ -810,6 +815,19 @@ int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_
     permute_out(h, xo.data(), x);
     if (res) *res = r;
     return 0;
+}
+
+extern "C" {
+
+int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
+                    int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res) {
+    return solve_mixed_common(h, b, x, tol, max_outer, inner_tol, inner_maxit, vec32, adaptive, eta, false, res);
+}
+
+int orc_adjoint_solve_mixed(orc_handle h, const double *w, double *lam, double tol, int32_t max_outer,
+                            double inner_tol, int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta,
+                            orc_result *res) {
+    return solve_mixed_common(h, w, lam, tol, max_outer, inner_tol, inner_maxit, vec32, adaptive, eta, true, res);
 }
 
 // dL/dA on the pattern, original numbering: grad_ij = -lambda_i x_j^T

@@ -60,6 +60,10 @@ int orc_solve(orc_handle h, const double *b, double *x, double tol, int32_t maxi
 int orc_solve_mixed(orc_handle h, const double *b, double *x, double tol, int32_t max_outer, double inner_tol,
                     int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta, orc_result *res);
 int orc_adjoint_solve(orc_handle h, const double *w, double *lam, double tol, int32_t maxit, orc_result *res);
+/* A^T lam = w in the precision form (the refinement of orc_solve_mixed on the transposed operators). */
+int orc_adjoint_solve_mixed(orc_handle h, const double *w, double *lam, double tol, int32_t max_outer,
+                            double inner_tol, int32_t inner_maxit, int32_t vec32, int32_t adaptive, double eta,
+                            orc_result *res);
 /* ||r0|| and every half-iteration's residual norm of the last orc_solve / orc_adjoint_solve. */
 int orc_get_history(orc_handle h, double *norms, int32_t capacity, int32_t *count);
 int orc_grad(orc_handle h, const double *lam, const double *x, double *grad);

@@ -129,6 +129,29 @@ def test_mixed_fixed_iterations(system):
             assert abs(rg.final_norm - ro["final_norm"]) <= dr + 1e-12 * ro["norms"][0]
 
 
+def test_adjoint_mixed_fixed_iterations(system):
+    """The precision-form adjoint, one refinement step with a fixed number of
+    inner iterations (inner tolerance 0), against the oracle's: the same
+    bounds as the forward form (the transposed FP32 kernels are bit-exact,
+    only the dot products' summation order differs)."""
+    _, s, g, o = system
+    A = sp.bsr_matrix((s.values, s.colidx, s.rowptr), shape=(s.N, s.N))
+    w = np.random.default_rng(26).standard_normal(s.N)
+    bounds = {1: 1e-12, 3: 1e-8, 6: 1e-7}
+    for vec32 in (False, True):
+        for k, bound in bounds.items():
+            lg, rg = g.adjoint_solve_mixed(w, tol=0.0, max_outer=1, inner_tol=0.0, inner_maxit=k, vec32=vec32)
+            lo, ro = o.adjoint_solve_mixed(w, tol=0.0, max_outer=1, inner_tol=0.0, inner_maxit=k, vec32=vec32)
+            assert rg.max_iter_reached and ro["max_iter_reached"]
+            assert rg.outer_iterations == ro["outer_iterations"] == 1
+            assert rg.inner_half_iterations == ro["inner_half_iterations"] == 2 * k
+            err = np.linalg.norm(lg - lo) / np.linalg.norm(lo)
+            print(f"adjoint vec32={vec32} inner_maxit={k}: lambda rel diff {err:.3e}", flush=True)
+            assert err <= bound, err
+            dr = np.linalg.norm(A.T @ (lg - lo))
+            assert abs(rg.final_norm - ro["final_norm"]) <= dr + 1e-12 * ro["norms"][0]
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2])
 def test_spmv_dev_on_caller_vectors(system, prec):
     """bda_spmv_dev on torch device tensors equals the oracle's product and

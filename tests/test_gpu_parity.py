@@ -292,3 +292,26 @@ def test_edge_watchdog_abort():
     assert r.converged and not r.aborted
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
     g.close()
+
+
+def test_adjoint_mixed_parity(system):
+    """The adjoint in the precision form: lambda against the oracle's FP64
+    adjoint (both to 1e-10) and the refinement step count against the
+    oracle's precision-form adjoint; the gradient it gives against the FP64
+    one."""
+    name, s, g, o = system
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(s.N)
+    lo, ro = o.adjoint_solve(w, tol=1e-10, maxit=500)
+    x, _ = o.solve(s.b, tol=1e-12, maxit=500)
+    for vec32 in (True, False):
+        lg, rg = g.adjoint_solve_mixed(w, tol=1e-10, vec32=vec32)
+        lom, rom = o.adjoint_solve_mixed(w, tol=1e-10, vec32=vec32)
+        print(name, "vec32" if vec32 else "vec64", rg.outer_iterations, rg.inner_half_iterations,
+              rom["outer_iterations"], rom["inner_half_iterations"], flush=True)
+        assert rg.converged and rom["converged"]
+        assert abs(rg.outer_iterations - rom["outer_iterations"]) <= 1
+        err = np.linalg.norm(lg - lo) / np.linalg.norm(lo)
+        assert err < 1e-8, err
+        gg, go = g.grad_values(lg, x), o.grad(lo, x)
+        assert np.linalg.norm(gg - go) <= 1e-8 * np.linalg.norm(go)
