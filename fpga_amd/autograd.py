@@ -45,8 +45,10 @@ class DifferentiableSolver:
         self.bs, self.n = bs, n
 
     def __call__(self, values: torch.Tensor, b: torch.Tensor, tol: float = 1e-10, maxit: int = 1000,
-                 adj_tol: float = 1e-10, adj_maxit: int = 1000) -> torch.Tensor:
-        return _ext.solve(values, b, self.solver.handle, tol, maxit, adj_tol, adj_maxit)
+                 adj_tol: float = 1e-10, adj_maxit: int = 1000, precision_form: bool = False) -> torch.Tensor:
+        """x = A(values)^-1 b.  precision_form: forward solve and adjoint in the
+        precision form (FP64 refinement around FP32-value inner solves)."""
+        return _ext.solve(values, b, self.solver.handle, tol, maxit, adj_tol, adj_maxit, int(precision_form))
 
     def close(self):
         self.solver.close()

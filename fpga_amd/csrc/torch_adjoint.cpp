@@ -41,9 +41,15 @@ void check_tensor(const Tensor &t, const char *name) {
     TORCH_CHECK(t.is_contiguous(), name, " must be contiguous");
 }
 
+constexpr int32_t kMaxOuter = 20;     // refinement steps of the precision form
+constexpr double kInnerTol = 1e-4;    // its inner tolerance
+
 struct BdaSolve : public torch::autograd::Function<BdaSolve> {
+    // precision_form != 0: the forward solve and the adjoint run in the
+    // precision form (FP64 refinement around FP32-value inner solves, inner
+    // tolerance 1e-4, maxit and adj_maxit bound each inner run).
     static Tensor forward(AutogradContext *ctx, Tensor values, Tensor b, int64_t handle, double tol, int64_t maxit,
-                          double adj_tol, int64_t adj_maxit) {
+                          double adj_tol, int64_t adj_maxit, int64_t precision_form) {
         bda_handle h = as_handle(handle);
         values = values.contiguous();
         b = b.contiguous();
@@ -54,8 +60,14 @@ struct BdaSolve : public torch::autograd::Function<BdaSolve> {
         check(h, bda_factor(h), "bda_factor");
         Tensor x = torch::zeros_like(b);
         bda_result res;
-        check(h, bda_solve_dev(h, b.data_ptr<double>(), x.data_ptr<double>(), tol, (int32_t)maxit, &res),
-              "bda_solve_dev");
+        if (precision_form)
+            check(h,
+                  bda_solve_mixed_dev(h, b.data_ptr<double>(), x.data_ptr<double>(), tol, kMaxOuter, kInnerTol,
+                                      (int32_t)maxit, 1, &res),
+                  "bda_solve_mixed_dev");
+        else
+            check(h, bda_solve_dev(h, b.data_ptr<double>(), x.data_ptr<double>(), tol, (int32_t)maxit, &res),
+                  "bda_solve_dev");
         TORCH_CHECK(res.converged || res.max_iter_reached, "forward solve failed (breakdown or NaN)");
         uint64_t gen = 0;
         check(h, bda_values_generation(h, &gen), "bda_values_generation");
@@ -64,6 +76,7 @@ struct BdaSolve : public torch::autograd::Function<BdaSolve> {
         ctx->saved_data["generation"] = (int64_t)gen;
         ctx->saved_data["adj_tol"] = adj_tol;
         ctx->saved_data["adj_maxit"] = adj_maxit;
+        ctx->saved_data["precision_form"] = precision_form;
         ctx->saved_data["nvalues"] = values.numel();
         ctx->saved_data["vshape"] = values.sizes().vec();
         return x;
@@ -87,21 +100,27 @@ struct BdaSolve : public torch::autograd::Function<BdaSolve> {
             check(h, bda_factor(h), "bda_factor");
         }
         bda_result res;
-        check(h,
-              bda_adjoint_solve_dev(h, w.data_ptr<double>(), lam.data_ptr<double>(),
-                                    ctx->saved_data["adj_tol"].toDouble(),
-                                    (int32_t)ctx->saved_data["adj_maxit"].toInt(), &res),
-              "bda_adjoint_solve_dev");
+        const double adj_tol = ctx->saved_data["adj_tol"].toDouble();
+        const int32_t adj_maxit = (int32_t)ctx->saved_data["adj_maxit"].toInt();
+        if (ctx->saved_data["precision_form"].toInt())
+            check(h,
+                  bda_adjoint_solve_mixed_dev(h, w.data_ptr<double>(), lam.data_ptr<double>(), adj_tol, kMaxOuter,
+                                              kInnerTol, adj_maxit, 1, &res),
+                  "bda_adjoint_solve_mixed_dev");
+        else
+            check(h, bda_adjoint_solve_dev(h, w.data_ptr<double>(), lam.data_ptr<double>(), adj_tol, adj_maxit, &res),
+                  "bda_adjoint_solve_dev");
         TORCH_CHECK(res.converged || res.no_results || res.max_iter_reached, "adjoint solve failed");
         Tensor gvals = torch::empty(ctx->saved_data["vshape"].toIntVector(), w.options());
         check(h, bda_grad_values_dev(h, lam.data_ptr<double>(), x.data_ptr<double>(), gvals.data_ptr<double>()),
               "bda_grad_values_dev");
-        return {gvals, lam, Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+        return {gvals, lam, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
     }
 };
 
-Tensor solve(Tensor values, Tensor b, int64_t handle, double tol, int64_t maxit, double adj_tol, int64_t adj_maxit) {
-    return BdaSolve::apply(values, b, handle, tol, maxit, adj_tol, adj_maxit);
+Tensor solve(Tensor values, Tensor b, int64_t handle, double tol, int64_t maxit, double adj_tol, int64_t adj_maxit,
+             int64_t precision_form) {
+    return BdaSolve::apply(values, b, handle, tol, maxit, adj_tol, adj_maxit, precision_form);
 }
 
 }  // namespace
@@ -109,5 +128,6 @@ Tensor solve(Tensor values, Tensor b, int64_t handle, double tol, int64_t maxit,
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
     m.doc() = "ILU0-BiCGSTAB solve with a hand-written adjoint (torch::autograd::Function)";
     m.def("solve", &solve, "x = A(values)^-1 b, differentiable in values and b", py::arg("values"), py::arg("b"),
-          py::arg("handle"), py::arg("tol"), py::arg("maxit"), py::arg("adj_tol"), py::arg("adj_maxit"));
+          py::arg("handle"), py::arg("tol"), py::arg("maxit"), py::arg("adj_tol"), py::arg("adj_maxit"),
+          py::arg("precision_form") = 0);
 }

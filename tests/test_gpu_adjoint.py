@@ -148,3 +148,24 @@ def test_backward_after_second_forward():
     assert np.linalg.norm(b2.grad.cpu().numpy() - lam2) / np.linalg.norm(lam2) < 1e-10
     np.testing.assert_allclose(v2.grad.cpu().numpy(), g2, rtol=1e-8, atol=1e-10 * np.abs(g2).max())
     ds.close()
+
+
+def test_autograd_precision_form_matches_fp64():
+    """The autograd Function in the precision form (forward and adjoint with
+    FP64 refinement around FP32-value inner solves, both to 1e-10): x and
+    both gradients within 1e-8 of the FP64 form's."""
+    from fpga_amd.configs import GRID_128x128x64, scaled
+    s = fa.generate(scaled(GRID_128x128x64, 20, 18, 10))
+    dev = torch.device("cuda:0")
+    ds = DifferentiableSolver(s.bs, s.rowptr, s.colidx, s.values)
+    w = torch.tensor(np.random.default_rng(3).standard_normal(s.N), device=dev)
+    out = []
+    for pf in (False, True):
+        vals = torch.tensor(s.values, device=dev, requires_grad=True)
+        b = torch.tensor(s.b, device=dev, requires_grad=True)
+        x = ds(vals, b, tol=1e-10, adj_tol=1e-10, precision_form=pf)
+        (w * x).sum().backward()
+        out.append((x.detach().cpu().numpy(), b.grad.cpu().numpy(), vals.grad.cpu().numpy()))
+    for ref, got in zip(out[0], out[1]):
+        assert np.linalg.norm(got - ref) <= 1e-8 * np.linalg.norm(ref)
+    ds.close()
