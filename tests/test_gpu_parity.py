@@ -7,6 +7,7 @@ at equal iteration counts (FP64).
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import fpga_amd as fa
 import oracle
@@ -315,3 +316,31 @@ def test_adjoint_mixed_parity(system):
         assert err < 1e-8, err
         gg, go = g.grad_values(lg, x), o.grad(lo, x)
         assert np.linalg.norm(gg - go) <= 1e-8 * np.linalg.norm(go)
+
+
+def test_adjoint_mixed_edge_cases():
+    """Stop rules of the precision-form adjoint against the oracle's, and
+    forward and adjoint runs of both forms back to back on one handle."""
+    s, g, o = _pair(scaled(GRID_32x32x16, 8, 7, 5))
+    w = np.random.default_rng(9).standard_normal(s.N)
+    lam, r = g.adjoint_solve_mixed(np.zeros(s.N), tol=1e-10)
+    assert r.no_results and not r.converged and r.outer_iterations == 0 and np.all(lam == 0)
+    lam, r = g.adjoint_solve_mixed(w, tol=1e-10, max_outer=0)
+    lo, ro = o.adjoint_solve_mixed(w, tol=1e-10, max_outer=0)
+    assert r.max_iter_reached and ro["max_iter_reached"] and r.outer_iterations == 0
+    assert np.all(lam == 0) and np.isclose(r.norms[0], ro["norms"][0], rtol=1e-12)
+    lam, r = g.adjoint_solve_mixed(w, tol=1e-14, max_outer=1)
+    lo, ro = o.adjoint_solve_mixed(w, tol=1e-14, max_outer=1)
+    assert r.max_iter_reached and ro["max_iter_reached"] and r.outer_iterations == ro["outer_iterations"] == 1
+    assert np.linalg.norm(lam - lo) / np.linalg.norm(lo) < 1e-3
+    l64, _ = o.adjoint_solve(w, tol=1e-12, maxit=500)
+    for _ in range(2):
+        xm, rm = g.solve_mixed(s.b, tol=1e-8)
+        lm, ra = g.adjoint_solve_mixed(w, tol=1e-10)
+        lf, rf = g.adjoint_solve(w, tol=1e-10, maxit=500)
+        assert rm.converged and ra.converged and rf.converged
+        assert np.linalg.norm(lm - l64) / np.linalg.norm(l64) < 1e-8
+        assert np.linalg.norm(lf - l64) / np.linalg.norm(l64) < 1e-8
+        assert np.linalg.norm(g.residual() - (w - sp.bsr_matrix((s.values, s.colidx, s.rowptr),
+                                                                  shape=(s.N, s.N)).T @ lf)) < 1e-6 * np.linalg.norm(w)
+    g.close()
