@@ -150,12 +150,16 @@ def test_backward_after_second_forward():
     ds.close()
 
 
-def test_autograd_precision_form_matches_fp64():
+@pytest.mark.parametrize("bs", [3, 4])
+def test_autograd_precision_form_matches_fp64(bs):
     """The autograd Function in the precision form (forward and adjoint with
     FP64 refinement around FP32-value inner solves, both to 1e-10): x and
     both gradients within 1e-8 of the FP64 form's."""
     from fpga_amd.configs import GRID_128x128x64, scaled
-    s = fa.generate(scaled(GRID_128x128x64, 20, 18, 10))
+    spec = scaled(GRID_128x128x64, 20, 18, 10) if bs == 3 else \
+        fa.GridSpec(14, 12, 8, bs=4, seed=2, perm_sigma=2.0, inactive_frac=0.1, nnc_frac=0.05, nnc_dist=3)
+    print(f"case {spec}", flush=True)
+    s = fa.generate(spec)
     dev = torch.device("cuda:0")
     ds = DifferentiableSolver(s.bs, s.rowptr, s.colidx, s.values)
     w = torch.tensor(np.random.default_rng(3).standard_normal(s.N), device=dev)

@@ -146,10 +146,25 @@ def test_level_scheduling_solve():
     assert rg.converged and ro["converged"] and abs(rg.half_iterations - ro["half_iterations"]) <= 2
     if rg.half_iterations == ro["half_iterations"]:
         assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-8
-    # the level-set sweeps bit for bit, as for the colouring
+    # the level-set sweeps bit for bit, as for the colouring, in every kernel
+    # form and transposed
     p = np.random.default_rng(9).uniform(-1, 1, s.N)
     np.testing.assert_array_equal(g.precond_apply_perm(p), o.precond_perm(p))
     np.testing.assert_array_equal(g.spmv_perm(p), o.spmv_perm(p))
+    for prec, dt in ((1, np.float32), (2, np.float64)):
+        for tr in (False, True):
+            np.testing.assert_array_equal(g.precond_apply_perm_ex(p.astype(dt), prec, transpose=tr),
+                                          o.precond_perm_ex(p.astype(dt), prec, transpose=tr))
+    # the precision form and both adjoints on the level-set ordering
+    xm, rm = g.solve_mixed(s.b, tol=1e-8)
+    assert rm.converged and np.linalg.norm(xm - xo) / np.linalg.norm(xo) < 1e-6
+    w = np.random.default_rng(10).standard_normal(s.N)
+    lo, ro = o.adjoint_solve(w, tol=1e-10, maxit=500)
+    lg, rg = g.adjoint_solve(w, tol=1e-10, maxit=500)
+    lm, rma = g.adjoint_solve_mixed(w, tol=1e-10)
+    assert rg.converged and rma.converged
+    assert np.linalg.norm(lg - lo) / np.linalg.norm(lo) < 1e-8
+    assert np.linalg.norm(lm - lo) / np.linalg.norm(lo) < 1e-8
     g.close()
 
 
