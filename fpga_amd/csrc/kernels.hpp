@@ -1610,6 +1610,10 @@ __global__ void k_gather_blocks(int64_t nblk, const int32_t *idx, const double *
 //   Dinv_i = inverse of the updated A_ii (Gauss-Jordan, partial pivoting)
 template <int BS>
 __device__ inline bool invert_block(const double *a, double *inv) {
+    // every loop unrolled and the pivot swap done with selects over the
+    // rows, so that m and v stay in registers (a swap that indexes rows by
+    // the pivot puts them in scratch memory); the arithmetic and its order
+    // are those of the textbook elimination
     double m[BS][BS], v[BS][BS];
 #pragma unroll
     for (int r = 0; r < BS; ++r)
@@ -1618,22 +1622,34 @@ __device__ inline bool invert_block(const double *a, double *inv) {
             m[r][c] = a[r * BS + c];
             v[r][c] = r == c ? 1.0 : 0.0;
         }
+#pragma unroll
     for (int c = 0; c < BS; ++c) {
         int piv = c;
         double best = fabs(m[c][c]);
+#pragma unroll
         for (int r = c + 1; r < BS; ++r)
             if (fabs(m[r][c]) > best) { best = fabs(m[r][c]); piv = r; }
         if (!(best > 0.0)) return false;
-        if (piv != c)
+#pragma unroll
+        for (int r = c + 1; r < BS; ++r) {
+            const bool sw = piv == r;
+#pragma unroll
             for (int q = 0; q < BS; ++q) {
-                double t = m[c][q]; m[c][q] = m[piv][q]; m[piv][q] = t;
-                t = v[c][q]; v[c][q] = v[piv][q]; v[piv][q] = t;
+                const double mc = m[c][q], mr = m[r][q], vc = v[c][q], vr = v[r][q];
+                m[c][q] = sw ? mr : mc;
+                m[r][q] = sw ? mc : mr;
+                v[c][q] = sw ? vr : vc;
+                v[r][q] = sw ? vc : vr;
             }
+        }
         const double p = m[c][c];
+#pragma unroll
         for (int q = 0; q < BS; ++q) { m[c][q] = m[c][q] / p; v[c][q] = v[c][q] / p; }
+#pragma unroll
         for (int r = 0; r < BS; ++r) {
             if (r == c) continue;
             const double f = m[r][c];
+#pragma unroll
             for (int q = 0; q < BS; ++q) {
                 m[r][q] = m[r][q] - f * m[c][q];
                 v[r][q] = v[r][q] - f * v[c][q];
