@@ -16,6 +16,9 @@ JSON line.
 Beside the headline the line carries, measured in the same run:
   trisolve   back-to-back preconditioner applications (all L and U sweep
              launches), FP64 and FP32, against their algorithmic bytes;
+  level_scheduling  the same system ordered by level sets instead of colours:
+             FP64 and precision-form solves, one preconditioner application
+             in each form and its launch count, beside the coloured figures;
   adjoint    the 64x64x32 grid's adjoint backward (A^T lambda = w to 1e-10,
              then the gradient -lambda x^T on the pattern), FP64 and in the
              precision form (with lambda's difference from the FP64 one);
@@ -58,7 +61,7 @@ def parse():
                          "counter runs")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the adjoint (64x64x32) and 4x4 (256x256x64) blocks")
+                    help="skip the level-scheduling, adjoint (64x64x32) and 4x4 (256x256x64) blocks")
     return ap.parse_args()
 
 
@@ -201,8 +204,11 @@ def main():
     ncolours = g.num_colours
     g.close()
 
-    extra_adj = extra_big = None
+    extra_adj = extra_big = extra_level = None
     if not args.no_extra:
+        extra_level = level_block(fa, torch, dev, args.workload, s, b, x_ref, tol, maxit, min(args.steps, 3),
+                                  {"fp64": (t_fp64, iters64, trisolve["fp64"]["avg_ms"]),
+                                   "mixed": (t_mixed, rmx.iterations, trisolve["fp32"]["avg_ms"])})
         extra_adj = adjoint_block(fa, torch, dev, min(args.steps, 5))
         extra_big = big_block(fa, torch, dev, min(args.steps, 3))
 
@@ -248,6 +254,7 @@ def main():
                      "frac_of_measured_read_ceiling": spmv32_gbs / read_ceiling},
         "trisolve": trisolve,
         "setup_s": {"generate": t_gen, "analyse": t_an, "factor": t_fac, "fp32_copies": t_dem},
+        "level_scheduling": extra_level,
         "adjoint": extra_adj,
         "grid256x256x64": extra_big,
         "cpu_baseline": cpu,
@@ -284,6 +291,59 @@ def setup(fa, spec, torch):
     g.factor()
     t_fac = time.perf_counter() - t0
     return s, g, {"generate": t_gen, "analyse": t_an, "factor": t_fac}
+
+
+def level_block(fa, torch, dev, name, s, b, x_ref, tol, maxit, steps, coloured):
+    """The same system and right-hand side with level scheduling instead of
+    the colouring (the host API's level_scheduling flag,
+    fpga_functions_bicgstab.cpp:84): one level set per sweep launch, so the
+    preconditioner costs one launch per level and sweep, against fewer
+    iterations from the natural-order ILU0.  FP64 and precision-form solves
+    to the same tolerance, their errors against the same 1e-12 solution, and
+    one preconditioner application back to back in each form."""
+    g = fa.BicgstabSolver(s.bs, s.n, s.nnzb)
+    g.upload_system(s)
+    t0 = time.perf_counter()
+    g.analyse(level_scheduling=True)
+    t_an = time.perf_counter() - t0
+    g.factor()
+    g.prepare_mixed()
+    sizes = np.diff(g.colouring()[2])
+    x = torch.zeros_like(x_ref)
+    nref = torch.linalg.vector_norm(x_ref).item()
+
+    def timed(fn):
+        x.zero_()
+        fn()
+        times, r = [], None
+        for _ in range(steps):
+            x.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)), r, torch.linalg.vector_norm(x - x_ref).item() / nref
+
+    t64, r64, e64 = timed(lambda: g.solve_dev(b.data_ptr(), x.data_ptr(), tol, maxit))
+    tmx, rmx, emx = timed(lambda: g.solve_mixed_dev(b.data_ptr(), x.data_ptr(), tol, 20, 1e-4, maxit, True))
+    pre64, pre32 = g.time_precond(5, False), g.time_precond(5, True)
+    launches = g.precond_launches()
+    g.close()
+    out = {"workload": f"{name}, same system, level sets of the lower-triangular graph",
+           "levels": int(sizes.size), "rows_per_level": {"max": int(sizes.max()), "median": int(np.median(sizes)),
+                                                         "min": int(sizes.min())},
+           "analyse_s": t_an, "precond_launches_per_application": launches, "steps": steps,
+           "fp64": {"time_s": t64, "iterations": r64.iterations, "converged": r64.converged,
+                    "x_rel_err_vs_1e-12": e64, "precond_application_ms": pre64},
+           "mixed": {"time_s": tmx, "outer": rmx.outer_iterations, "inner_iterations": rmx.iterations,
+                     "converged": rmx.converged, "x_rel_err_vs_1e-12": emx, "precond_application_ms": pre32},
+           "coloured": {k: {"time_s": v[0], "iterations": v[1], "precond_application_ms": v[2]}
+                        for k, v in coloured.items()}}
+    log(f"level scheduling {name}: {sizes.size} levels, fp64 {t64 * 1e3:.1f} ms ({r64.iterations} it), "
+        f"mixed {tmx * 1e3:.1f} ms ({rmx.iterations} inner it), precond {pre64:.3f} / {pre32:.3f} ms, "
+        f"{launches} launches")
+    return out
 
 
 def adjoint_block(fa, torch, dev, steps):
@@ -350,6 +410,7 @@ def big_block(fa, torch, dev, steps):
     nref = torch.linalg.vector_norm(x_ref).item()
 
     def timed(fn):
+        x.zero_()
         fn()
         times, r = [], None
         for _ in range(steps):
