@@ -168,10 +168,16 @@ class OracleSystem:
         if rc != 0:
             raise RuntimeError("orc_create failed")
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "_h", None):
             lib().orc_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass    # at interpreter exit the module's globals may already be gone
 
     @property
     def N(self):
