@@ -404,13 +404,17 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_one_pass(RedArgs ra, i
     if (ra.st->done) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double v[NRED][U];
+    // clamped loads, all in flight together, and a select afterwards (a
+    // load under a branch gets a wait of its own before the branch joins)
 #pragma unroll
     for (int j = 0; j < NRED; ++j)
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int i = u * kFinThreads + threadIdx.x;
-            v[j][u] = i < nparts ? ra.partials[(size_t)j * ra.cap + i] : 0.0;
-        }
+        for (int u = 0; u < U; ++u)
+            v[j][u] = ra.partials[(size_t)j * ra.cap + min(u * kFinThreads + (int)threadIdx.x, max(nparts - 1, 0))];
+#pragma unroll
+    for (int j = 0; j < NRED; ++j)
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[j][u] = u * kFinThreads + (int)threadIdx.x < nparts ? v[j][u] : 0.0;
 #pragma unroll
     for (int j = 0; j < NRED; ++j) {
         double t = 0.0 + v[j][0];
@@ -430,6 +434,64 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize_one_pass(RedArgs ra, i
         finalize(ra.st, ra.fin, t[0], t[1]);
     }
 }
+
+// k_finalize<NRED, kFinUnroll> for at most kFinThreads * (kFinUnroll + U2)
+// partials (the 3x3 128x128x64 grid's SpMV writes 18,019): the loads of both
+// passes and of all NRED reductions are issued together (k_finalize waits
+// for each pass of each reduction in turn), then summed exactly as
+// k_finalize sums them.  Its second-pass loads past the first U2 read +0.0
+// for every thread, and a running sum that started as 0.0 + v is never -0.0,
+// so leaving those additions out leaves every sum bit for bit the same.
+template <int NRED, int U2>
+__global__ __launch_bounds__(kFinThreads) void k_finalize_joint(RedArgs ra, int nparts) {
+    static_assert(U2 < kFinUnroll, "a short second pass");
+    __shared__ double sred[kFinThreads / 64][2];
+    if (ra.st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int A = kFinUnroll;
+    double v0[NRED][A], v1[NRED][U2];
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+        const double *pp = ra.partials + (size_t)j * ra.cap;
+        // clamped loads and a select afterwards: a load under a branch
+        // gets a wait of its own before the branch joins
+#pragma unroll
+        for (int u = 0; u < A; ++u) v0[j][u] = pp[min(u * kFinThreads + (int)threadIdx.x, nparts - 1)];
+#pragma unroll
+        for (int u = 0; u < U2; ++u) v1[j][u] = pp[min((A + u) * kFinThreads + (int)threadIdx.x, nparts - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+#pragma unroll
+        for (int u = 0; u < A; ++u) v0[j][u] = u * kFinThreads + (int)threadIdx.x < nparts ? v0[j][u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U2; ++u) v1[j][u] = (A + u) * kFinThreads + (int)threadIdx.x < nparts ? v1[j][u] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < NRED; ++j) {
+        double a8[A];
+#pragma unroll
+        for (int u = 0; u < A; ++u) a8[u] = 0.0 + v0[j][u];
+#pragma unroll
+        for (int u = 0; u < U2; ++u) a8[u] += v1[j][u];
+        double t = a8[0];
+#pragma unroll
+        for (int u = 1; u < A; ++u) t += a8[u];
+        const double acc = wave_sum(t);
+        if (lane == 0) sred[wave][j] = acc;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[2] = {0.0, 0.0};
+        for (int j = 0; j < NRED; ++j) {
+            t[j] = sred[0][j];
+            for (int w = 1; w < kFinThreads / 64; ++w) t[j] += sred[w][j];
+        }
+        finalize(ra.st, ra.fin, t[0], t[1]);
+    }
+}
+constexpr int kFinJointU2 = 2;
+constexpr int kFinJointTo = kFinThreads * (kFinUnroll + kFinJointU2);
 
 // Above kFinTwoStageFrom partials (the 4x4 256x256x64 grid's SpMV writes
 // 95,044) one workgroup reads them at about 55 GB/s; a first stage of

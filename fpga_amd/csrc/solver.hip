@@ -525,6 +525,11 @@ enum ProfKind { PK_SPMV = 0, PK_LOWER = 1, PK_UPPER = 2, PK_VEC = 3, PK_N = 4 };
 #ifndef BDA_LOW2
 #define BDA_LOW2 1
 #endif
+// 1 (default): k_finalize_joint for 2,049 to 18,432 partials; 0: k_finalize
+// (the same sums, for A/B timing)
+#ifndef BDA_FIN_JOINT
+#define BDA_FIN_JOINT 1
+#endif
 
 struct bda_handle_s {
     int bs = 0, n = 0;
@@ -719,6 +724,9 @@ void launch_finalize(H &h, int nred, const RedArgs &ra, int nparts) {
     } else if (nparts <= 2 * kFinThreads) {
         if (nred == 1) hipLaunchKernelGGL((k_finalize_one_pass<1, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
         else hipLaunchKernelGGL((k_finalize_one_pass<2, 2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+    } else if (BDA_FIN_JOINT && nparts <= kFinJointTo) {
+        if (nred == 1) hipLaunchKernelGGL((k_finalize_joint<1, kFinJointU2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
+        else hipLaunchKernelGGL((k_finalize_joint<2, kFinJointU2>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
     } else if (nred == 1) {
         if (wide) hipLaunchKernelGGL((k_finalize<1, kFinUnrollWide>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
         else hipLaunchKernelGGL((k_finalize<1, kFinUnroll>), dim3(1), dim3(kFinThreads), 0, h.stream, ra, nparts);
