@@ -11,7 +11,7 @@ import scipy.sparse as sp
 
 import fpga_amd as fa
 import oracle
-from fpga_amd.configs import GRID_32x32x16, GRID_128x128x64, scaled
+from fpga_amd.configs import GRID_32x32x16, GRID_128x128x64, GRID_256x256x64, scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -137,8 +137,11 @@ def test_adjoint_parity(system):
     np.testing.assert_array_equal(g.grad_values(lo, x), o.grad(lo, x))
 
 
-def test_level_scheduling_solve():
-    s, g, o = _pair(scaled(GRID_128x128x64, 12, 10, 8), level=True)
+@pytest.mark.parametrize("bs", [3, 4])
+def test_level_scheduling_solve(bs):
+    base = GRID_128x128x64 if bs == 3 else GRID_256x256x64
+    s, g, o = _pair(scaled(base, 12, 10, 8), level=True)
+    assert s.bs == bs
     for a, b in zip(g.colouring(), o.colouring()):
         np.testing.assert_array_equal(a, b)
     xg, rg = g.solve(s.b, tol=1e-8, maxit=200)
