@@ -55,10 +55,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
-    ap.add_argument("--mode", choices=["full", "kernels"], default="full",
+    ap.add_argument("--mode", choices=["full", "kernels", "level"], default="full",
                     help="kernels: only repeated SpMV launches and preconditioner applications "
                          "(FP64 and FP32 values) and a reference copy of known size, for rocprofv3 "
-                         "counter runs")
+                         "counter runs; level: only the level-scheduling block (its JSON), for a "
+                         "kernel trace of the level-set launches")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the level-scheduling, adjoint (64x64x32) and 4x4 (256x256x64) blocks")
@@ -114,6 +115,11 @@ def main():
     g.solve_dev(b.data_ptr(), x.data_ptr(), 1e-12, 5000)
     x_ref = x.clone()
     nref = torch.linalg.vector_norm(x_ref).item()
+    if args.mode == "level":
+        g.close()
+        print(json.dumps(level_block(fa, torch, dev, args.workload, s, b, x_ref, tol, maxit, args.steps, None)),
+              flush=True)
+        return 0
 
     def run_fp64():
         x.zero_()
@@ -338,7 +344,8 @@ def level_block(fa, torch, dev, name, s, b, x_ref, tol, maxit, steps, coloured):
                     "x_rel_err_vs_1e-12": e64, "precond_application_ms": pre64},
            "mixed": {"time_s": tmx, "outer": rmx.outer_iterations, "inner_iterations": rmx.iterations,
                      "converged": rmx.converged, "x_rel_err_vs_1e-12": emx, "precond_application_ms": pre32},
-           "coloured": {k: {"time_s": v[0], "iterations": v[1], "precond_application_ms": v[2]}
+           "coloured": None if coloured is None else
+                       {k: {"time_s": v[0], "iterations": v[1], "precond_application_ms": v[2]}
                         for k, v in coloured.items()}}
     log(f"level scheduling {name}: {sizes.size} levels, fp64 {t64 * 1e3:.1f} ms ({r64.iterations} it), "
         f"mixed {tmx * 1e3:.1f} ms ({rmx.iterations} inner it), precond {pre64:.3f} / {pre32:.3f} ms, "
